@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: effective FP64 GFLOPS (2mnk/t) of 2-level <2,2,2;7> Strassen at
+16384^3 on one MI355X, beside the project's classical FP64 MFMA GEMM.
+
+One step = one C := A*B + C product through the C ABI (fmmhip_dgemm_fmm) on
+device-resident, seeded uniform[-1,1] operands (column-major).  W untimed
+warm-up steps, then K timed steps between device synchronisations; every
+launch is also bracketed by HIP events on the library's stream to give the
+kernel's average duration (roofline) and its latency percentiles.
+
+Prints ONE JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fmm-blis_amd"))
+
+FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 matrix peak
+FP32_PEAK_TFLOPS = 157.3
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def leaf_flops(info, m, n, k):
+    lm = -(-m // info["mt"])
+    lk = -(-k // info["kt"])
+    ln = -(-n // info["nt"])
+    return 2.0 * info["R"] * lm * lk * ln
+
+
+def cpu_baseline(seconds_target=15.0):
+    """The oracle's 1-level Strassen (222) at 2048^3 on the host cores: the
+    CPU port of the reference's path (the reference itself is not built)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    n = 2048
+    A, B, C = O.uniform(n, n, 0, 0), O.uniform(n, n, 0, 1), O.uniform(n, n, 0, 2)
+    alg = O.parse(os.path.join(ROOT, "tests", "golden", "algorithms", "222.txt"))
+    times = []
+    t_end = time.time() + seconds_target
+    while True:
+        Cw = C.copy(order="F")
+        t0 = time.perf_counter()
+        O.fmm_dgemm(alg, 1.0, A, B, 1.0, Cw)
+        times.append(time.perf_counter() - t0)
+        if time.time() > t_end or len(times) >= 3:
+            break
+    t = min(times)
+    return {"value": round(2.0 * n ** 3 / t / 1e9, 2), "unit": "GFLOPS", "cores": O.threads(), "kind": "port",
+            "sample": f"oracle 1-level <2,2,2;7> Strassen FP64, 2048^3, C += A*B, min of {len(times)} runs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=16384)
+    ap.add_argument("--alg", default="222")
+    ap.add_argument("--levels", type=int, default=2)
+    ap.add_argument("--leaf", default="f64", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--classical-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU sharding is a "
+              f"design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import fmm_blis_amd as F
+    F.check(F.lib().fmmhip_init(), "init")
+    n = args.size
+    m = k = n
+    if args.alg == "222":
+        f = F.strassen()
+    else:
+        f = F.new_fmm(os.path.join(ROOT, "tests", "golden", "algorithms", f"{args.alg}.txt"))
+    alg = F.Alg([f] * args.levels)
+    info = alg.info()
+    leaf = {"f64": F.LEAF_NATIVE, "f32": F.LEAF_F32, "bf16x3": F.LEAF_BF16X3}[args.leaf]
+
+    A = F.DeviceMatrix(m, k).fill_uniform(0, 0)
+    B = F.DeviceMatrix(k, n).fill_uniform(0, 1)
+    C = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+    C0 = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+    F.sync()
+
+    def step():
+        F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, C, leaf=leaf)
+
+    for _ in range(args.warmup):
+        step()
+    F.sync()
+    evs = [(F.Event(), F.Event()) for _ in range(args.steps)]
+    F.sync()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record()
+        step()
+        e1.record()
+    F.sync()
+    wall = time.perf_counter() - t0
+    lat = [e0.elapsed_ms(e1) for e0, e1 in evs]
+    ms_step = wall * 1e3 / args.steps
+    kern_ms = sum(lat) / len(lat)
+    eff = 2.0 * m * n * k / (ms_step * 1e-3) / 1e9
+
+    # classical FP64 MFMA GEMM on the same operands (its own C)
+    Cc = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+    F.dgemm(m, n, k, 1.0, A, B, 1.0, Cc)
+    F.sync()
+    ce = [(F.Event(), F.Event()) for _ in range(args.classical_steps)]
+    for e0, e1 in ce:
+        e0.record()
+        F.dgemm(m, n, k, 1.0, A, B, 1.0, Cc)
+        e1.record()
+    F.sync()
+    cls_ms = sum(e0.elapsed_ms(e1) for e0, e1 in ce) / len(ce)
+    cls_gflops = 2.0 * m * n * k / (cls_ms * 1e-3) / 1e9
+
+    # accuracy: one fresh FMM and one fresh classical product from the same C0
+    Cf = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+    Cr = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+    F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, Cf, leaf=leaf)
+    F.dgemm(m, n, k, 1.0, A, B, 1.0, Cr)
+    F.sync()
+    rel = F.fro_diff(Cf, Cr) / (F.fro(A) * F.fro(B))
+
+    fl = leaf_flops(info, m, n, k)
+    peak = FP64_PEAK_TFLOPS if args.leaf == "f64" else FP32_PEAK_TFLOPS
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+    out = {
+        "metric": "effective FP64 GFLOPS (2mnk/t) FMM vs classical at 16384^3; p99 GEMM latency",
+        "value": round(eff, 1),
+        "unit": "GFLOPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.leaf == "f64" else f"f64 ({args.leaf} leaf)",
+        "data": "synthetic (SplitMix64 uniform[-1,1], seed 0, device-resident)",
+        "config": {
+            "workload": f"{args.levels}-level <{f.m_tilde},{f.k_tilde},{f.n_tilde};{f.R}> FMM, FP64 C += A*B, "
+                        f"m=n=k={n}, column-major",
+            "m": m, "n": n, "k": k, "alg": args.alg, "levels": args.levels, "leaf": args.leaf,
+            "composed": [info["mt"], info["kt"], info["nt"], info["R"]],
+            "classical_fp64_gflops": round(cls_gflops, 1),
+            "fmm_over_classical": round(eff / cls_gflops, 4),
+            "rel_err_vs_classical": rel,
+        },
+        "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3),
+                       "mean": round(kern_ms, 3)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "fmm_gemm_kernel", "leaf_flops_per_launch": fl,
+                     "effective_over_peak": round(eff / 1e3 / peak, 4)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+/*
+ * fmm_internal.h -- declarations shared by the host C files (algorithm
+ * tables, geometry, the obj_t driver) and the HIP layer (kernels and their
+ * launchers).  Not installed; include/fmm_blis_amd.h is the public ABI.
+ */
+#ifndef FMM_INTERNAL_H
+#define FMM_INTERNAL_H
+
+#include <stdint.h>
+#include "../../include/fmm_blis_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One nonzero coefficient of a product, resolved against a problem's
+ * geometry: the part starts at element offset `off` of its operand, has
+ * `rows` x `cols` valid elements (the rest of the leaf is zero padding),
+ * and enters with coefficient `coef`.  For C entries, `first` marks the
+ * first product that writes the part (where beta is applied). */
+typedef struct fmm_entry_s {
+    int64_t off;
+    int32_t rows;
+    int32_t cols;
+    double  coef;
+    int32_t first;
+    int32_t part;
+} fmm_entry_t;
+
+/* Composed algorithm (levels already multiplied out) plus, per product r,
+ * the compressed lists of its nonzero U, V, W coefficients. */
+struct fmmhip_alg_s {
+    int  mt, kt, nt, R, levels;
+    int  classical;             /* R == 1, 1x1x1 */
+    int *U, *V, *W;             /* dense [part][r] */
+    int *u_ptr, *u_part, *u_coef;   /* CSR over r: entries u_ptr[r]..u_ptr[r+1]-1 */
+    int *v_ptr, *v_part, *v_coef;
+    int *w_ptr, *w_part, *w_coef, *w_first;
+    long nnz_u, nnz_v, nnz_w;
+    int  max_nnz_u, max_nnz_v, max_nnz_w;
+    void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
+};
+
+/* Problem geometry of one call (host side). */
+typedef struct fmm_geom_s {
+    long m, n, k;               /* user dims of C (m x n) and the inner k */
+    long lm, lk, ln;            /* leaf dims: ceil(m/mt), ceil(k/kt), ceil(n/nt) */
+    long rsA, csA, rsB, csB, rsC, csC;
+} fmm_geom_t;
+
+/* Part split of `whole` into `tilde` contiguous parts, sizes
+ * floor(whole/tilde) + (t < whole mod tilde), offsets the prefix sums
+ * (frame/base/bli_strassen_ab.c:86-121). */
+void fmm_part_split(long whole, int tilde, long* size, long* off);
+
+/* Resolves the algorithm's coefficient lists against geometry g into
+ * host arrays (caller frees with free()):  ptr[3*(R+1)] = u, v, w CSR
+ * pointers; ent = u entries, then v, then w.  Returns 0 or an fmmhip
+ * status. */
+int fmm_build_entries(const struct fmmhip_alg_s* alg, const fmm_geom_t* g,
+                      int32_t** ptr_out, fmm_entry_t** ent_out, long* nent_out);
+
+/* HIP layer hooks used by the C files. */
+void fmmhip_alg_release_cache(struct fmmhip_alg_s* alg);
+/* Scales C by beta (beta == 0 writes zeros without reading C). */
+int fmmhip_scale_d(double* C, long m, long n, long rsC, long csC, double beta, void* stream);
+int fmmhip_scale_s(float* C, long m, long n, long rsC, long csC, float beta, void* stream);
+/* Pointer kind: 1 device (or managed), 0 host, <0 error. */
+int fmmhip_pointer_is_device(const void* p);
+/* The library's default stream (created on first use). */
+void* fmmhip_default_stream(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+// fmmhip_runtime.hip -- device set-up, memory, streams, events, seeded fills
+// and reductions behind the fmmhip_* C ABI (include/fmm_blis_amd.h).
+//
+// The reference has no device layer; its counterparts are BLIS's own memory
+// and thread set-up (blis/frame/base/bli_init.c, bli_obj.c).  Errors are
+// returned as fmmhip_status_t codes, never by aborting.
+#include <hip/hip_runtime.h>
+#include <mutex>
+#include <string.h>
+#include <stdio.h>
+
+#include "fmm_internal.h"
+
+static std::once_flag g_init_once;
+static int g_init_status = FMMHIP_ERR_NODEVICE;
+static hipStream_t g_stream = nullptr;
+static int g_cus = 0;
+static char g_arch[64] = {0};
+
+static void do_init()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_init_status = FMMHIP_ERR_NODEVICE; return; }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) { g_init_status = FMMHIP_ERR_HIP; return; }
+    strncpy(g_arch, prop.gcnArchName, sizeof(g_arch) - 1);
+    g_cus = prop.multiProcessorCount;
+    if (strncmp(g_arch, "gfx950", 6) != 0) {
+        fprintf(stderr, "fmm_blis_amd: device arch '%s' is not gfx950\n", g_arch);
+        g_init_status = FMMHIP_ERR_NODEVICE;
+        return;
+    }
+    if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess) {
+        g_init_status = FMMHIP_ERR_HIP;
+        return;
+    }
+    g_init_status = FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_init(void)
+{
+    std::call_once(g_init_once, do_init);
+    return g_init_status;
+}
+
+extern "C" void* fmmhip_default_stream(void)
+{
+    fmmhip_init();
+    return (void*)g_stream;
+}
+
+extern "C" const char* fmmhip_status_string(int s)
+{
+    switch (s) {
+    case FMMHIP_SUCCESS: return "success";
+    case FMMHIP_ERR_INVALID: return "invalid argument";
+    case FMMHIP_ERR_NOMEM: return "out of memory";
+    case FMMHIP_ERR_HIP: return "HIP runtime error";
+    case FMMHIP_ERR_UNSUPPORTED: return "unsupported";
+    case FMMHIP_ERR_NODEVICE: return "no gfx950 device";
+    case FMMHIP_ERR_IO: return "coefficient file unreadable or malformed";
+    default: return "unknown status";
+    }
+}
+
+extern "C" int fmmhip_device_arch(char* buf, size_t len)
+{
+    int rc = fmmhip_init();
+    if (buf && len) { strncpy(buf, g_arch, len - 1); buf[len - 1] = 0; }
+    return rc;
+}
+
+extern "C" int fmmhip_device_cu_count(int* cus)
+{
+    int rc = fmmhip_init();
+    if (cus) *cus = g_cus;
+    return rc;
+}
+
+#define HIP_OK(x) do { if ((x) != hipSuccess) return FMMHIP_ERR_HIP; } while (0)
+#define INIT_OR_RETURN() do { int _s = fmmhip_init(); if (_s) return _s; } while (0)
+
+static inline hipStream_t S(void* s) { return s ? (hipStream_t)s : g_stream; }
+
+extern "C" int fmmhip_malloc(void** p, size_t bytes)
+{
+    INIT_OR_RETURN();
+    if (!p) return FMMHIP_ERR_INVALID;
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) { *p = nullptr; (void)hipGetLastError(); return FMMHIP_ERR_NOMEM; }
+    return FMMHIP_SUCCESS;
+}
+extern "C" int fmmhip_free(void* p) { INIT_OR_RETURN(); HIP_OK(hipFree(p)); return 0; }
+extern "C" int fmmhip_memcpy_h2d(void* d, const void* s, size_t b) { INIT_OR_RETURN(); HIP_OK(hipMemcpy(d, s, b, hipMemcpyHostToDevice)); return 0; }
+extern "C" int fmmhip_memcpy_d2h(void* d, const void* s, size_t b) { INIT_OR_RETURN(); HIP_OK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost)); return 0; }
+extern "C" int fmmhip_memset(void* p, int v, size_t b) { INIT_OR_RETURN(); HIP_OK(hipMemset(p, v, b)); return 0; }
+extern "C" int fmmhip_stream_create(void** s)
+{
+    INIT_OR_RETURN();
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = (void*)st;
+    return 0;
+}
+extern "C" int fmmhip_stream_destroy(void* s) { INIT_OR_RETURN(); HIP_OK(hipStreamDestroy((hipStream_t)s)); return 0; }
+extern "C" int fmmhip_stream_sync(void* s) { INIT_OR_RETURN(); HIP_OK(hipStreamSynchronize(S(s))); return 0; }
+extern "C" int fmmhip_device_sync(void) { INIT_OR_RETURN(); HIP_OK(hipDeviceSynchronize()); return 0; }
+extern "C" int fmmhip_event_create(void** e)
+{
+    INIT_OR_RETURN();
+    hipEvent_t ev;
+    HIP_OK(hipEventCreate(&ev));
+    *e = (void*)ev;
+    return 0;
+}
+extern "C" int fmmhip_event_destroy(void* e) { INIT_OR_RETURN(); HIP_OK(hipEventDestroy((hipEvent_t)e)); return 0; }
+extern "C" int fmmhip_event_record(void* e, void* s) { INIT_OR_RETURN(); HIP_OK(hipEventRecord((hipEvent_t)e, S(s))); return 0; }
+extern "C" int fmmhip_event_elapsed_ms(float* ms, void* a, void* b)
+{
+    INIT_OR_RETURN();
+    HIP_OK(hipEventSynchronize((hipEvent_t)b));
+    HIP_OK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return 0;
+}
+extern "C" int fmmhip_mem_info(size_t* f, size_t* t)
+{
+    INIT_OR_RETURN();
+    HIP_OK(hipMemGetInfo(f, t));
+    return 0;
+}
+
+extern "C" int fmmhip_pointer_is_device(const void* p)
+{
+    if (fmmhip_init()) return -1;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) ? 1 : 0;
+}
+
+// ---- seeded fills (bit-identical to the CPU rule) --------------------------
+
+__device__ __forceinline__ uint64_t smix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t counter_bits(uint64_t base, uint64_t idx)
+{
+    return smix(base + (idx + 1) * 0x9E3779B97F4A7C15ull);
+}
+
+static uint64_t stream_base(uint64_t seed, uint64_t sid)
+{
+    uint64_t z = seed ^ (sid * 0xD1B54A32D192ED03ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <int MODE, typename T>
+__global__ void fill_kernel(T* X, long m, long n, long ld, uint64_t base, int amp)
+{
+    long total = m * n;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long j = idx / m, i = idx - j * m;
+        uint64_t z = counter_bits(base, (uint64_t)idx) >> 11;
+        double v;
+        if (MODE == 0) v = 2.0 * ((double)z * 0x1.0p-53) - 1.0;
+        else v = (double)((long)(z % (uint64_t)(2 * amp + 1)) - amp);
+        X[i + j * ld] = (T)v;
+    }
+}
+
+static int grid_for(long total)
+{
+    long g = (total + 255) / 256;
+    return (int)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+extern "C" int fmmhip_fill_uniform(double* X, long m, long n, long ld, uint64_t seed, uint64_t sid, void* s)
+{
+    INIT_OR_RETURN();
+    if (m < 0 || n < 0 || ld < m || (m * n && !X)) return FMMHIP_ERR_INVALID;
+    if (!m || !n) return 0;
+    hipLaunchKernelGGL((fill_kernel<0, double>), dim3(grid_for(m * n)), dim3(256), 0, S(s), X, m, n, ld, stream_base(seed, sid), 0);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+extern "C" int fmmhip_fill_uniform_f32(float* X, long m, long n, long ld, uint64_t seed, uint64_t sid, void* s)
+{
+    INIT_OR_RETURN();
+    if (m < 0 || n < 0 || ld < m || (m * n && !X)) return FMMHIP_ERR_INVALID;
+    if (!m || !n) return 0;
+    hipLaunchKernelGGL((fill_kernel<0, float>), dim3(grid_for(m * n)), dim3(256), 0, S(s), X, m, n, ld, stream_base(seed, sid), 0);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+extern "C" int fmmhip_fill_int(double* X, long m, long n, long ld, uint64_t seed, uint64_t sid, int amp, void* s)
+{
+    INIT_OR_RETURN();
+    if (m < 0 || n < 0 || ld < m || amp < 0 || (m * n && !X)) return FMMHIP_ERR_INVALID;
+    if (!m || !n) return 0;
+    hipLaunchKernelGGL((fill_kernel<1, double>), dim3(grid_for(m * n)), dim3(256), 0, S(s), X, m, n, ld, stream_base(seed, sid), amp);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+__global__ void scale_rc_kernel(double* X, long m, long n, long ld, const double* rs, const double* cs)
+{
+    long total = m * n;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long j = idx / m, i = idx - j * m;
+        double v = X[i + j * ld];
+        if (rs) v *= rs[i];
+        if (cs) v *= cs[j];
+        X[i + j * ld] = v;
+    }
+}
+
+extern "C" int fmmhip_scale_rows_cols(double* X, long m, long n, long ld, const double* rs, const double* cs, void* s)
+{
+    INIT_OR_RETURN();
+    if (m < 0 || n < 0 || ld < m) return FMMHIP_ERR_INVALID;
+    if (!m || !n) return 0;
+    hipLaunchKernelGGL(scale_rc_kernel, dim3(grid_for(m * n)), dim3(256), 0, S(s), X, m, n, ld, rs, cs);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// C <- beta*C over a general-strided m x n matrix; beta == 0 writes zeros.
+template <typename T>
+__global__ void scale_kernel(T* C, long m, long n, long rs, long cs, T beta)
+{
+    long total = m * n;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long j = idx / m, i = idx - j * m;
+        T* p = C + i * rs + j * cs;
+        *p = beta == (T)0 ? (T)0 : beta * *p;
+    }
+}
+
+extern "C" int fmmhip_scale_d(double* C, long m, long n, long rs, long cs, double beta, void* s)
+{
+    INIT_OR_RETURN();
+    if (!m || !n || beta == 1.0) return 0;
+    hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for(m * n)), dim3(256), 0, S(s), C, m, n, rs, cs, beta);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+extern "C" int fmmhip_scale_s(float* C, long m, long n, long rs, long cs, float beta, void* s)
+{
+    INIT_OR_RETURN();
+    if (!m || !n || beta == 1.0f) return 0;
+    hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(m * n)), dim3(256), 0, S(s), C, m, n, rs, cs, beta);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// ---- Frobenius norms (FP64, deterministic two-stage reduction) -------------
+
+template <bool DIFF>
+__global__ void fro_kernel(const double* X, const double* Y, long m, long n, long ldx, long ldy, double* part)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    long total = m * n;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long j = idx / m, i = idx - j * m;
+        double d = X[i + j * ldx];
+        if (DIFF) d -= Y[i + j * ldy];
+        s += d * d;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+static int fro_impl(const double* X, const double* Y, long m, long n, long ldx, long ldy, double* out, bool diff)
+{
+    INIT_OR_RETURN();
+    if (!out || m < 0 || n < 0) return FMMHIP_ERR_INVALID;
+    *out = 0.0;
+    if (!m || !n) return 0;
+    HIP_OK(hipDeviceSynchronize());   // X, Y may come from any stream
+    const int nb = 1024;
+    double* part = nullptr;
+    if (hipMalloc(&part, nb * sizeof(double)) != hipSuccess) return FMMHIP_ERR_NOMEM;
+    if (diff) hipLaunchKernelGGL(fro_kernel<true>, dim3(nb), dim3(256), 0, g_stream, X, Y, m, n, ldx, ldy, part);
+    else hipLaunchKernelGGL(fro_kernel<false>, dim3(nb), dim3(256), 0, g_stream, X, Y, m, n, ldx, ldy, part);
+    double h[nb];
+    int rc = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess) rc = FMMHIP_ERR_HIP;
+    (void)hipFree(part);
+    if (rc) return rc;
+    double s = 0.0;
+    for (int i = 0; i < nb; i++) s += h[i];
+    *out = sqrt(s);
+    return 0;
+}
+
+extern "C" int fmmhip_fro_diff(const double* X, const double* Y, long m, long n, long ldx, long ldy, double* out)
+{
+    return fro_impl(X, Y, m, n, ldx, ldy, out, true);
+}
+extern "C" int fmmhip_fro(const double* X, long m, long n, long ld, double* out)
+{
+    return fro_impl(X, nullptr, m, n, ld, ld, out, false);
+}

@@ -1,0 +1,343 @@
+"""ctypes binding of libfmm_blis_amd.so (include/fmm_blis_amd.h).
+
+This is the Python face of the C ABI: the fmm_blis drop-in (fmm_t, obj_t,
+new_fmm, bli_strassen_ab_ex, ...) and the native fmmhip_* layer (device
+buffers, streams, events, seeded fills, the FMM and classical GEMMs).  Tests
+and bench.py call the library only through here.  Loading fails loudly when
+the shared library has not been built; there is no fallback path.
+
+Reference interface mirrored (dnparikh/fmm-blis): fmm_t
+plugin/bli_plugin_fmm_blis.h:58-68, new_fmm/free_fmm
+frame/base/bli_read_fmm.c:24-86, bli_strassen_ab_ex/bli_strassen_ab
+frame/base/bli_strassen_ab.c:123-300, bli_plugin_register_fmm_blis
+plugin/bli_plugin_register.c:43-64.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfmm_blis_amd.so")
+
+# Datatype codes of obj_t.info bits 0-2 (blis/frame/include/bli_type_defs.h:378-381)
+BLIS_FLOAT = 0
+BLIS_DOUBLE = 2
+BLIS_SUCCESS = -1
+BLIS_FAILURE = -2
+TRANS_BIT = 1 << 3
+
+LEAF_NATIVE, LEAF_F32, LEAF_BF16X3 = 0, 1, 2
+
+STATUS = {0: "success", 1: "invalid argument", 2: "out of memory", 3: "HIP runtime error",
+          4: "unsupported", 5: "no gfx950 device", 6: "coefficient file unreadable or malformed"}
+
+
+class FmmT(ctypes.Structure):
+    """fmm_t: <m~, k~, n~; R> with int tables U, V, W (row-major [part][r])."""
+    _fields_ = [("m_tilde", ctypes.c_int), ("n_tilde", ctypes.c_int), ("k_tilde", ctypes.c_int),
+                ("R", ctypes.c_int), ("U", ctypes.POINTER(ctypes.c_int)),
+                ("V", ctypes.POINTER(ctypes.c_int)), ("W", ctypes.POINTER(ctypes.c_int))]
+
+    def tables(self):
+        """(U, V, W) as nested Python lists [part][r]."""
+        mt, kt, nt, R = self.m_tilde, self.k_tilde, self.n_tilde, self.R
+        def grab(p, parts):
+            return [[p[s * R + r] for r in range(R)] for s in range(parts)]
+        return grab(self.U, mt * kt), grab(self.V, kt * nt), grab(self.W, mt * nt)
+
+
+class _Atom(ctypes.Structure):
+    _fields_ = [("real", ctypes.c_double), ("imag", ctypes.c_double)]
+
+
+class ObjT(ctypes.Structure):
+    """Layout-identical mirror of BLIS obj_t (bli_type_defs.h:1215-1245)."""
+    pass
+
+
+ObjT._fields_ = [("root", ctypes.POINTER(ObjT)), ("off", ctypes.c_int64 * 2), ("dim", ctypes.c_int64 * 2),
+                 ("diag_off", ctypes.c_int64), ("info", ctypes.c_uint32), ("info2", ctypes.c_uint32),
+                 ("elem_size", ctypes.c_uint64), ("buffer", ctypes.c_void_p), ("rs", ctypes.c_int64),
+                 ("cs", ctypes.c_int64), ("is_", ctypes.c_int64), ("scalar", _Atom),
+                 ("m_padded", ctypes.c_int64), ("n_padded", ctypes.c_int64), ("ps", ctypes.c_int64),
+                 ("pd", ctypes.c_int64), ("m_panel", ctypes.c_int64), ("n_panel", ctypes.c_int64)]
+
+
+class FmmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_L = ctypes.c_long
+_D = ctypes.c_double
+_F = ctypes.c_float
+_I = ctypes.c_int
+_U64 = ctypes.c_uint64
+
+_SIGS = {
+    # drop-in
+    "bli_plugin_register_fmm_blis": (_I, []),
+    "new_fmm": (FmmT, [ctypes.c_char_p]),
+    "free_fmm": (None, [FmmT]),
+    "bli_strassen_ab_ex": (None, [ctypes.POINTER(ObjT)] * 5 + [FmmT]),
+    "bli_strassen_ab": (None, [ctypes.POINTER(ObjT)] * 5),
+    "bli_fmm_ab_ex": (None, [ctypes.POINTER(ObjT)] * 5 + [ctypes.POINTER(FmmT), _I, _I]),
+    "bli_gemm_hip": (None, [ctypes.POINTER(ObjT)] * 5),
+    "fmm_blis_amd_set_abort_on_error": (None, [_I]),
+    "fmm_blis_amd_last_error": (_I, []),
+    "fmm_obj_attach": (None, [ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, _P, ctypes.c_int64,
+                              ctypes.c_int64, ctypes.POINTER(ObjT)]),
+    "fmm_obj_scalar": (None, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ObjT)]),
+    # native
+    "fmmhip_status_string": (ctypes.c_char_p, [_I]),
+    "fmmhip_init": (_I, []),
+    "fmmhip_device_arch": (_I, [ctypes.c_char_p, ctypes.c_size_t]),
+    "fmmhip_device_cu_count": (_I, [ctypes.POINTER(_I)]),
+    "fmmhip_alg_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(FmmT), _I]),
+    "fmmhip_alg_create_classical": (_I, [ctypes.POINTER(_P)]),
+    "fmmhip_alg_destroy": (None, [_P]),
+    "fmmhip_alg_info": (_I, [_P] + [ctypes.POINTER(_I)] * 4 + [ctypes.POINTER(_L)] * 3),
+    "fmmhip_alg_brent_violations": (_L, [_P]),
+    "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
+    "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
+    "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
+    "fmmhip_sgemm": (_I, [_L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
+    "fmmhip_malloc": (_I, [ctypes.POINTER(_P), ctypes.c_size_t]),
+    "fmmhip_free": (_I, [_P]),
+    "fmmhip_memcpy_h2d": (_I, [_P, _P, ctypes.c_size_t]),
+    "fmmhip_memcpy_d2h": (_I, [_P, _P, ctypes.c_size_t]),
+    "fmmhip_memset": (_I, [_P, _I, ctypes.c_size_t]),
+    "fmmhip_stream_create": (_I, [ctypes.POINTER(_P)]),
+    "fmmhip_stream_destroy": (_I, [_P]),
+    "fmmhip_stream_sync": (_I, [_P]),
+    "fmmhip_device_sync": (_I, []),
+    "fmmhip_event_create": (_I, [ctypes.POINTER(_P)]),
+    "fmmhip_event_destroy": (_I, [_P]),
+    "fmmhip_event_record": (_I, [_P, _P]),
+    "fmmhip_event_elapsed_ms": (_I, [ctypes.POINTER(_F), _P, _P]),
+    "fmmhip_mem_info": (_I, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "fmmhip_fill_uniform": (_I, [_P, _L, _L, _L, _U64, _U64, _P]),
+    "fmmhip_fill_uniform_f32": (_I, [_P, _L, _L, _L, _U64, _U64, _P]),
+    "fmmhip_fill_int": (_I, [_P, _L, _L, _L, _U64, _U64, _I, _P]),
+    "fmmhip_scale_rows_cols": (_I, [_P, _L, _L, _L, _P, _P, _P]),
+    "fmmhip_fro_diff": (_I, [_P, _P, _L, _L, _L, _L, ctypes.POINTER(_D)]),
+    "fmmhip_fro": (_I, [_P, _L, _L, _L, ctypes.POINTER(_D)]),
+}
+
+EXPORTED = sorted(_SIGS) + ["STRASSEN_FMM", "FMM_BLIS_PACK_UKR", "FMM_BLIS_GEMM_UKR"]
+
+
+def lib():
+    """Loads the shared library (once).  Raises FmmError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FmmError(f"{LIB_PATH} is not built: run __graft_entry__.build() (make -C fmm-blis_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().fmmhip_status_string(rc).decode()
+        raise FmmError(f"{what}: {msg} ({rc})")
+
+
+def strassen():
+    """The library's built-in STRASSEN_FMM (frame/base/bli_strassen_ab.c:12-24)."""
+    return FmmT.in_dll(lib(), "STRASSEN_FMM")
+
+
+def new_fmm(path):
+    return lib().new_fmm(os.fsencode(path))
+
+
+def free_fmm(f):
+    lib().free_fmm(f)
+
+
+class Alg:
+    """Composed L-level algorithm (fmmhip_alg_t)."""
+
+    def __init__(self, levels=None, classical=False):
+        self.h = _P()
+        if classical:
+            check(lib().fmmhip_alg_create_classical(ctypes.byref(self.h)), "alg_create_classical")
+        else:
+            arr = (FmmT * len(levels))(*levels)
+            check(lib().fmmhip_alg_create(ctypes.byref(self.h), arr, len(levels)), "alg_create")
+
+    def info(self):
+        mt, kt, nt, R = _I(), _I(), _I(), _I()
+        nu, nv, nw = _L(), _L(), _L()
+        check(lib().fmmhip_alg_info(self.h, *(ctypes.byref(x) for x in (mt, kt, nt, R, nu, nv, nw))))
+        return dict(mt=mt.value, kt=kt.value, nt=nt.value, R=R.value, nnz_u=nu.value, nnz_v=nv.value,
+                    nnz_w=nw.value)
+
+    def brent_violations(self):
+        return lib().fmmhip_alg_brent_violations(self.h)
+
+    def close(self):
+        if self.h:
+            lib().fmmhip_alg_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Raw device allocation through fmmhip_malloc."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = _P()
+        check(lib().fmmhip_malloc(ctypes.byref(self.ptr), max(self.nbytes, 1)), f"malloc {nbytes}")
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def upload(self, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(lib().fmmhip_memcpy_h2d(self.ptr, a.ctypes.data_as(_P), a.nbytes), "h2d")
+
+    def download(self, dtype, count):
+        import numpy as np
+        out = np.empty(count, dtype=dtype)
+        check(lib().fmmhip_memcpy_d2h(out.ctypes.data_as(_P), self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().fmmhip_free(self.ptr)
+            self.ptr = _P()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceMatrix:
+    """Column-major m x n matrix in device memory (leading dimension ld)."""
+
+    def __init__(self, m, n, dtype="f8", ld=None):
+        import numpy as np
+        self.m, self.n = int(m), int(n)
+        self.dtype = np.dtype(dtype)
+        self.ld = int(ld) if ld is not None else max(self.m, 1)
+        self.buf = DeviceBuffer(self.dtype.itemsize * self.ld * max(self.n, 1))
+
+    @property
+    def ptr(self):
+        return self.buf.ptr
+
+    def fill_uniform(self, seed, stream_id, stream=None):
+        f = lib().fmmhip_fill_uniform if self.dtype.itemsize == 8 else lib().fmmhip_fill_uniform_f32
+        check(f(self.ptr, self.m, self.n, self.ld, seed, stream_id, stream), "fill_uniform")
+        return self
+
+    def fill_int(self, seed, stream_id, amp, stream=None):
+        assert self.dtype.itemsize == 8
+        check(lib().fmmhip_fill_int(self.ptr, self.m, self.n, self.ld, seed, stream_id, amp, stream), "fill_int")
+        return self
+
+    def upload(self, arr):
+        import numpy as np
+        a = np.asarray(arr, dtype=self.dtype)
+        full = np.zeros((self.n, self.ld), dtype=self.dtype)
+        full[:, : self.m] = a.T
+        self.buf.upload(full)
+        return self
+
+    def to_numpy(self):
+        check(lib().fmmhip_device_sync(), "sync")
+        flat = self.buf.download(self.dtype, self.ld * max(self.n, 1))
+        return flat.reshape(max(self.n, 1), self.ld)[: self.n, : self.m].T.copy()
+
+    def free(self):
+        self.buf.free()
+
+
+def dgemm_fmm(alg, m, n, k, alpha, A, B, beta, C, leaf=LEAF_NATIVE, stream=None):
+    """C := beta*C + alpha*A*B on column-major DeviceMatrix operands."""
+    check(lib().fmmhip_dgemm_fmm(alg.h, leaf, m, n, k, alpha, A.ptr, 1, A.ld, B.ptr, 1, B.ld, beta,
+                                 C.ptr, 1, C.ld, stream), "dgemm_fmm")
+
+
+def sgemm_fmm(alg, m, n, k, alpha, A, B, beta, C, stream=None):
+    check(lib().fmmhip_sgemm_fmm(alg.h, m, n, k, alpha, A.ptr, 1, A.ld, B.ptr, 1, B.ld, beta,
+                                 C.ptr, 1, C.ld, stream), "sgemm_fmm")
+
+
+def dgemm(m, n, k, alpha, A, B, beta, C, stream=None):
+    check(lib().fmmhip_dgemm(m, n, k, alpha, A.ptr, 1, A.ld, B.ptr, 1, B.ld, beta, C.ptr, 1, C.ld, stream),
+          "dgemm")
+
+
+def sgemm(m, n, k, alpha, A, B, beta, C, stream=None):
+    check(lib().fmmhip_sgemm(m, n, k, alpha, A.ptr, 1, A.ld, B.ptr, 1, B.ld, beta, C.ptr, 1, C.ld, stream),
+          "sgemm")
+
+
+def fro_diff(X, Y):
+    out = _D()
+    check(lib().fmmhip_fro_diff(X.ptr, Y.ptr, X.m, X.n, X.ld, Y.ld, ctypes.byref(out)), "fro_diff")
+    return out.value
+
+
+def fro(X):
+    out = _D()
+    check(lib().fmmhip_fro(X.ptr, X.m, X.n, X.ld, ctypes.byref(out)), "fro")
+    return out.value
+
+
+def sync():
+    check(lib().fmmhip_device_sync(), "device_sync")
+
+
+class Event:
+    def __init__(self):
+        self.h = _P()
+        check(lib().fmmhip_event_create(ctypes.byref(self.h)), "event_create")
+
+    def record(self, stream=None):
+        check(lib().fmmhip_event_record(self.h, stream), "event_record")
+
+    def elapsed_ms(self, later):
+        ms = _F()
+        check(lib().fmmhip_event_elapsed_ms(ctypes.byref(ms), self.h, later.h), "event_elapsed")
+        return ms.value
+
+
+def obj_attach(arr_or_ptr, m, n, rs, cs, dt=BLIS_DOUBLE):
+    """obj_t view of a numpy array (host) or an integer device address."""
+    o = ObjT()
+    if hasattr(arr_or_ptr, "ctypes"):
+        p = arr_or_ptr.ctypes.data_as(_P)
+    else:
+        p = _P(arr_or_ptr)
+    lib().fmm_obj_attach(dt, m, n, p, rs, cs, ctypes.byref(o))
+    return o
+
+
+def obj_scalar(value):
+    """1x1 double scalar object; returns (obj, storage) -- keep storage alive."""
+    st = ctypes.c_double(value)
+    o = ObjT()
+    lib().fmm_obj_scalar(ctypes.byref(st), ctypes.byref(o))
+    return o, st

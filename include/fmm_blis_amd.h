@@ -1,0 +1,280 @@
+/*
+ * fmm_blis_amd.h -- C ABI of the MI355X fast-matrix-multiplication library
+ * (libfmm_blis_amd.so).
+ *
+ * Two layers:
+ *
+ *  1. The fmm_blis drop-in.  Same names, argument meaning and types as the
+ *     plugin and driver of dnparikh/fmm-blis, so a program written against
+ *     libfmm.a + libblis_fmm_blis.a relinks against this library unchanged:
+ *       bli_plugin_register_fmm_blis  plugin/bli_plugin_register.c:43-64
+ *       FMM_BLIS_PACK_UKR/GEMM_UKR    plugin/bli_plugin_fmm_blis.h:45-46
+ *       fmm_t                         plugin/bli_plugin_fmm_blis.h:58-68
+ *       new_fmm / free_fmm            frame/base/bli_read_fmm.c:24-86
+ *       STRASSEN_FMM                  frame/base/bli_strassen_ab.c:12-24
+ *       bli_strassen_ab_ex            frame/base/bli_strassen_ab.c:123-295
+ *       bli_strassen_ab               frame/base/bli_strassen_ab.c:297-300
+ *     The operands are BLIS objects (obj_t, blis/frame/include/
+ *     bli_type_defs.h:1215-1245).  When blis.h is included first its obj_t is
+ *     used; otherwise a layout-identical declaration follows.
+ *
+ *  2. The native layer (fmmhip_*): plain pointers, sizes and strides, device
+ *     resident operands, HIP streams passed as void*.  The drop-in calls it;
+ *     tests, benchmarks, the batching and the serving forms call it directly.
+ *
+ * All entry points are C (extern "C"), no C++ or HIP types appear here.
+ */
+#ifndef FMM_BLIS_AMD_H
+#define FMM_BLIS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* BLIS types (layout-identical mirror unless blis.h is already included)   */
+/* ------------------------------------------------------------------------ */
+#ifndef BLIS_H
+typedef int64_t  dim_t;
+typedef int64_t  inc_t;
+typedef int64_t  doff_t;
+typedef uint64_t siz_t;
+typedef uint32_t objbits_t;
+typedef int      err_t;      /* BLIS err_t is an int-sized enum */
+typedef uint32_t num_t;
+
+typedef struct { double real; double imag; } fmm_atom_t;   /* BLIS atom_t */
+
+typedef struct obj_s {
+    struct obj_s* root;
+    dim_t         off[2];
+    dim_t         dim[2];
+    doff_t        diag_off;
+    objbits_t     info;
+    objbits_t     info2;
+    siz_t         elem_size;
+    void*         buffer;
+    inc_t         rs;
+    inc_t         cs;
+    inc_t         is;
+    fmm_atom_t    scalar;
+    dim_t         m_padded;
+    dim_t         n_padded;
+    inc_t         ps;
+    inc_t         pd;
+    dim_t         m_panel;
+    dim_t         n_panel;
+} obj_t;
+
+#define BLIS_SUCCESS  (-1)
+#define BLIS_FAILURE  (-2)
+#define BLIS_FLOAT    0x0u
+#define BLIS_SCOMPLEX 0x1u
+#define BLIS_DOUBLE   0x2u
+#define BLIS_DCOMPLEX 0x3u
+#define BLIS_INT      0x4u
+#define BLIS_CONSTANT 0x5u
+#endif /* BLIS_H */
+
+/* Bits of obj_t.info this library reads (bli_type_defs.h:291-340). */
+#define FMM_OBJ_DT_MASK    0x7u          /* datatype, bits 0-2 */
+#define FMM_OBJ_TRANS_BIT  (0x1u << 3)   /* op(X) = X^T        */
+#define FMM_OBJ_CONJ_BIT   (0x1u << 4)   /* no-op for real data */
+
+/* ------------------------------------------------------------------------ */
+/* fmm_blis drop-in                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Algorithm descriptor <m~, k~, n~; R> with integer coefficient tables
+ * U[(i*k~+p)*R + r], V[(p*n~+j)*R + r], W[(i*n~+j)*R + r].
+ * Replaces plugin/bli_plugin_fmm_blis.h:58-68 (same field order). */
+typedef struct fmm_s {
+    int  m_tilde;
+    int  n_tilde;
+    int  k_tilde;
+    int  R;
+    int* U;
+    int* V;
+    int* W;
+} fmm_t;
+
+/* Kernel slot ids.  Replaces plugin/bli_plugin_fmm_blis.h:45-46; set by
+ * bli_plugin_register_fmm_blis (the device kernels are reached through the
+ * library, not through a BLIS context). */
+extern siz_t FMM_BLIS_PACK_UKR;
+extern siz_t FMM_BLIS_GEMM_UKR;
+
+/* Initialises the HIP device and the kernel registry once (thread-safe).
+ * Returns BLIS_SUCCESS (-1), or BLIS_FAILURE (-2) when no usable gfx950
+ * device is present.  Replaces plugin/bli_plugin_register.c:43-64. */
+err_t bli_plugin_register_fmm_blis(void);
+
+/* Built-in <2,2,2;7> Strassen, identical to 222.txt.
+ * Replaces frame/base/bli_strassen_ab.c:12-24. */
+extern fmm_t STRASSEN_FMM;
+
+/* Reads a coefficient file: header "m k n R", then m*k rows of U, k*n rows
+ * of V, m*n rows of W (R integers each); lines starting with '#' are
+ * comments, line length is unbounded.  U/V/W are malloc'd and owned by the
+ * caller (free_fmm).  On a missing or malformed file a message goes to
+ * stderr and the result has R == 0 and NULL tables.
+ * Replaces frame/base/bli_read_fmm.c:24-80. */
+fmm_t new_fmm(const char* file_name);
+
+/* Frees the tables of a new_fmm result.  Replaces bli_read_fmm.c:82-86. */
+void free_fmm(fmm_t fmm);
+
+/* C := beta*C + alpha*op(A)*op(B) with one level of the algorithm fmm.
+ * Real single or double objects of one datatype; any of A, B, C may be
+ * row- or column-stored (or transposed through the obj_t trans bit); the
+ * buffers may be host memory (staged through the device) or device memory
+ * (computed in place).  Synchronous, like the reference.  Errors print a
+ * message and abort(), as BLIS does with error checking on; see
+ * fmm_blis_amd_set_abort_on_error.
+ * Replaces frame/base/bli_strassen_ab.c:123-295. */
+void bli_strassen_ab_ex(obj_t* alpha, obj_t* A, obj_t* B, obj_t* beta,
+                        obj_t* C, fmm_t fmm);
+
+/* bli_strassen_ab_ex(..., STRASSEN_FMM).
+ * Replaces frame/base/bli_strassen_ab.c:297-300. */
+void bli_strassen_ab(obj_t* alpha, obj_t* A, obj_t* B, obj_t* beta, obj_t* C);
+
+/* ------------------------------------------------------------------------ */
+/* Extensions of the drop-in (no counterpart in the reference)              */
+/* ------------------------------------------------------------------------ */
+
+/* Leaf-product arithmetic. */
+typedef enum {
+    FMMHIP_LEAF_NATIVE = 0,   /* the operands' type: FP64 for double, FP32 for float */
+    FMMHIP_LEAF_F32    = 1,   /* FP64 operands, FP32 MFMA leaf, FP64 writeback       */
+    FMMHIP_LEAF_BF16X3 = 2    /* FP64 operands, split BF16 MFMA leaf (FP32 accumulate),
+                                 FP64 recombination and writeback                     */
+} fmmhip_leaf_t;
+
+/* Like bli_strassen_ab_ex with `nlevels` levels; level l uses algs[l]
+ * (mixing algorithms is allowed) and leaf arithmetic `leaf`. */
+void bli_fmm_ab_ex(obj_t* alpha, obj_t* A, obj_t* B, obj_t* beta, obj_t* C,
+                   const fmm_t* algs, int nlevels, fmmhip_leaf_t leaf);
+
+/* Classical product C := beta*C + alpha*op(A)*op(B) through the same
+ * object boundary (the role of BLIS bli_gemm on this path). */
+void bli_gemm_hip(obj_t* alpha, obj_t* A, obj_t* B, obj_t* beta, obj_t* C);
+
+/* When off (default on) the object entries record errors instead of
+ * aborting; fmm_blis_amd_last_error returns the last one (0 = none). */
+void fmm_blis_amd_set_abort_on_error(int on);
+int  fmm_blis_amd_last_error(void);
+
+/* Fills *obj as a view of an existing buffer, like BLIS
+ * bli_obj_create_with_attached_buffer (blis/frame/base/bli_obj.c:59-75):
+ * dt BLIS_FLOAT or BLIS_DOUBLE, m x n, element (i,j) at p + i*rs + j*cs. */
+void fmm_obj_attach(num_t dt, dim_t m, dim_t n, void* p, inc_t rs, inc_t cs,
+                    obj_t* obj);
+/* Fills *obj as a 1x1 double scalar holding *storage (caller-owned). */
+void fmm_obj_scalar(double* storage, obj_t* obj);
+
+/* ------------------------------------------------------------------------ */
+/* Native layer                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef enum {
+    FMMHIP_SUCCESS         = 0,
+    FMMHIP_ERR_INVALID     = 1,  /* bad argument or shape                  */
+    FMMHIP_ERR_NOMEM       = 2,  /* host or device allocation failed       */
+    FMMHIP_ERR_HIP         = 3,  /* a HIP runtime call or a launch failed  */
+    FMMHIP_ERR_UNSUPPORTED = 4,  /* valid but not supported (e.g. complex) */
+    FMMHIP_ERR_NODEVICE    = 5,  /* no usable gfx950 device                */
+    FMMHIP_ERR_IO          = 6   /* coefficient file unreadable/malformed  */
+} fmmhip_status_t;
+
+const char* fmmhip_status_string(int status);
+
+/* Device set-up (idempotent, thread-safe); 0 on success. */
+int fmmhip_init(void);
+/* Name of the device's architecture (e.g. "gfx950") into buf. */
+int fmmhip_device_arch(char* buf, size_t len);
+int fmmhip_device_cu_count(int* cus);
+
+/* Composed algorithm: nlevels levels, level l = algs[l].  The tables are
+ * the Kronecker product of the levels (part (i1*m2+i2, p1*k2+p2), product
+ * r1*R2+r2).  Validates every level against the Brent equations. */
+typedef struct fmmhip_alg_s* fmmhip_alg_t;
+int  fmmhip_alg_create(fmmhip_alg_t* out, const fmm_t* algs, int nlevels);
+/* One level, R = 1, all coefficients 1: the classical product. */
+int  fmmhip_alg_create_classical(fmmhip_alg_t* out);
+void fmmhip_alg_destroy(fmmhip_alg_t alg);
+/* Composed <m~,k~,n~;R> and nonzero counts of U, V, W. */
+int  fmmhip_alg_info(fmmhip_alg_t alg, int* mt, int* kt, int* nt, int* R,
+                     long* nnz_u, long* nnz_v, long* nnz_w);
+/* Number of equations of the Brent system the tables violate (0 = exact). */
+long fmmhip_alg_brent_violations(fmmhip_alg_t alg);
+
+/* C := beta*C + alpha*A*B on device pointers, asynchronous on `stream`
+ * (NULL = the library's stream).  Element (i,j) of X is X[i*rsX + j*csX];
+ * for A and B one of rs, cs must be 1.  The leaf `leaf` applies to the
+ * double entry; the float entry always computes FP32 leaves. */
+int fmmhip_dgemm_fmm(fmmhip_alg_t alg, fmmhip_leaf_t leaf,
+                     long m, long n, long k, double alpha,
+                     const double* A, long rsA, long csA,
+                     const double* B, long rsB, long csB, double beta,
+                     double* C, long rsC, long csC, void* stream);
+int fmmhip_sgemm_fmm(fmmhip_alg_t alg, long m, long n, long k, float alpha,
+                     const float* A, long rsA, long csA,
+                     const float* B, long rsB, long csB, float beta,
+                     float* C, long rsC, long csC, void* stream);
+/* The project's classical FP64 / FP32 MFMA GEMMs (same kernels, R = 1). */
+int fmmhip_dgemm(long m, long n, long k, double alpha,
+                 const double* A, long rsA, long csA,
+                 const double* B, long rsB, long csB, double beta,
+                 double* C, long rsC, long csC, void* stream);
+int fmmhip_sgemm(long m, long n, long k, float alpha,
+                 const float* A, long rsA, long csA,
+                 const float* B, long rsB, long csB, float beta,
+                 float* C, long rsC, long csC, void* stream);
+
+/* Device memory, streams, events (thin wrappers so that C and ctypes
+ * callers need no HIP headers). */
+int fmmhip_malloc(void** p, size_t bytes);
+int fmmhip_free(void* p);
+int fmmhip_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int fmmhip_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int fmmhip_memset(void* p, int value, size_t bytes);
+int fmmhip_stream_create(void** stream);
+int fmmhip_stream_destroy(void* stream);
+int fmmhip_stream_sync(void* stream);
+int fmmhip_device_sync(void);
+int fmmhip_event_create(void** ev);
+int fmmhip_event_destroy(void* ev);
+int fmmhip_event_record(void* ev, void* stream);
+int fmmhip_event_elapsed_ms(float* ms, void* ev_start, void* ev_stop);
+/* Device memory currently free / total, in bytes. */
+int fmmhip_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* Seeded inputs on the device.  Element (i,j) of an m x n column-major
+ * matrix (leading dimension ld) gets the value of counter j*m+i of stream
+ * `stream_id` under `seed` (SplitMix64 -> 2u-1, u = 53-bit uniform), so the
+ * bits equal a CPU fill with the same rule.  fill_int draws integers in
+ * [-amp, amp] from the same counters. */
+int fmmhip_fill_uniform(double* X, long m, long n, long ld, uint64_t seed,
+                        uint64_t stream_id, void* stream);
+int fmmhip_fill_uniform_f32(float* X, long m, long n, long ld, uint64_t seed,
+                            uint64_t stream_id, void* stream);
+int fmmhip_fill_int(double* X, long m, long n, long ld, uint64_t seed,
+                    uint64_t stream_id, int amp, void* stream);
+/* Scales row i of X by rowscale[i] (if non-NULL) and column j by
+ * colscale[j] (if non-NULL); scales are device arrays. */
+int fmmhip_scale_rows_cols(double* X, long m, long n, long ld,
+                           const double* rowscale, const double* colscale,
+                           void* stream);
+/* ||X - Y||_F, ||X||_F computed on the device in FP64 (result to host). */
+int fmmhip_fro_diff(const double* X, const double* Y, long m, long n,
+                    long ldx, long ldy, double* out);
+int fmmhip_fro(const double* X, long m, long n, long ld, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMM_BLIS_AMD_H */

@@ -1,0 +1,275 @@
+"""GPU parity: the HIP path through the C ABI against the pinned CPU oracle
+on the same seeded inputs.
+
+Bars (stated per test):
+  * integer-valued inputs make every algorithm exact in FP64, so the device
+    result must equal the oracle's bit for bit (any indexing, sign, clipping
+    or beta mistake shows);
+  * uniform[-1,1] inputs: normalised error ||C - C_ref||_F/(||A||_F ||B||_F)
+    against the oracle's classical product, below the per-level bounds;
+  * the device fill reproduces the oracle's seeded bits exactly.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import fmm_blis_amd as F
+import oracle as O
+from conftest import ALG_NAMES, alg_path
+
+pytestmark = pytest.mark.gpu
+
+# normalised-error bounds for uniform[-1,1] inputs at the sizes below
+TOL_FP64 = {1: 1e-15, 2: 5e-15}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    rc = F.lib().fmmhip_init()
+    if rc != 0:
+        pytest.fail(f"no usable gfx950 device: {F.STATUS.get(rc)}")
+    buf = ctypes.create_string_buffer(64)
+    F.lib().fmmhip_device_arch(buf, 64)
+    say("device", buf.value.decode())
+    yield
+
+
+_algs = {}
+
+
+def fmm(name):
+    if name not in _algs:
+        _algs[name] = F.new_fmm(alg_path(name))
+    return _algs[name]
+
+
+def dev(X):
+    D = F.DeviceMatrix(X.shape[0], X.shape[1], dtype=X.dtype)
+    D.upload(X)
+    return D
+
+
+def run_fmm(alg, alpha, A, B, beta, C, leaf=F.LEAF_NATIVE):
+    m, k = A.shape
+    n = B.shape[1]
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    F.dgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC, leaf=leaf)
+    F.sync()
+    return dC.to_numpy()
+
+
+@pytest.mark.parametrize("m,n,seed,sid", [(1, 1, 0, 0), (37, 29, 0, 3), (1000, 17, 5, 1)])
+def test_device_fill_bit_identical(m, n, seed, sid):
+    say("fill", m, n, seed, sid)
+    D = F.DeviceMatrix(m, n).fill_uniform(seed, sid)
+    assert np.array_equal(D.to_numpy(), O.uniform(m, n, seed, sid))
+    I = F.DeviceMatrix(m, n).fill_int(seed, sid, 4)
+    assert np.array_equal(I.to_numpy(), O.integers(m, n, seed, sid, 4))
+    S = F.DeviceMatrix(m, n, dtype="f4").fill_uniform(seed, sid)
+    assert np.array_equal(S.to_numpy(), O.uniform_f32(m, n, seed, sid))
+
+
+CLASSICAL_SHAPES = [(1, 1, 1), (3, 13, 7), (128, 128, 16), (129, 130, 17), (200, 64, 333), (512, 384, 256),
+                    (1, 300, 5), (300, 1, 77), (33, 65, 1)]
+
+
+@pytest.mark.parametrize("m,n,k", CLASSICAL_SHAPES)
+def test_classical_dgemm_exact_integers(m, n, k):
+    say("classical int", m, n, k)
+    A, B, C = O.integers(m, k, 1, 0, 4), O.integers(k, n, 1, 1, 4), O.integers(m, n, 1, 2, 4)
+    alg = F.Alg(classical=True)
+    for alpha, beta in [(1.0, 1.0), (-2.0, 0.5), (1.0, 0.0)]:
+        want = O.dgemm(alpha, A, B, beta, C.copy(order="F"))
+        got = run_fmm(alg, alpha, A, B, beta, C)
+        assert np.array_equal(got, want), (alpha, beta)
+
+
+@pytest.mark.parametrize("m,n,k", [(257, 255, 300), (512, 512, 512)])
+def test_classical_dgemm_random(m, n, k):
+    say("classical rand", m, n, k)
+    A, B, C = O.uniform(m, k, 0, 0), O.uniform(k, n, 0, 1), O.uniform(m, n, 0, 2)
+    want = O.dgemm(1.0, A, B, 1.0, C.copy(order="F"))
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    F.dgemm(m, n, k, 1.0, dA, dB, 1.0, dC)
+    got = dC.to_numpy()
+    assert O.normalized_error(got, want, A, B) < 1e-15
+
+
+FMM_SHAPES = [(3, 13, 7), (31, 33, 35), (64, 64, 64), (97, 101, 103), (258, 260, 262), (1, 5, 3), (17, 1, 19),
+              (300, 280, 36)]
+
+
+@pytest.mark.parametrize("name", ALG_NAMES)
+@pytest.mark.parametrize("levels", [1, 2])
+def test_fmm_exact_integers(name, levels):
+    """Bit-exact agreement with the oracle's FMM on integer inputs (uneven
+    splits included)."""
+    f = fmm(name)
+    alg = F.Alg([f] * levels)
+    og = O.parse(alg_path(name))
+    og = og if levels == 1 else O.compose(og, og)
+    shapes = FMM_SHAPES if not (name == "444" and levels == 2) else [(20, 18, 17), (130, 140, 150)]
+    for (m, n, k) in shapes:
+        say("fmm int", name, levels, m, n, k)
+        A, B, C = O.integers(m, k, 3, 0, 4), O.integers(k, n, 3, 1, 4), O.integers(m, n, 3, 2, 4)
+        for alpha, beta in [(1.0, 1.0), (2.0, -3.0), (0.5, 0.0)]:
+            want = O.fmm_dgemm(og, alpha, A, B, beta, C.copy(order="F"))
+            got = run_fmm(alg, alpha, A, B, beta, C)
+            assert np.array_equal(got, want), (m, n, k, alpha, beta)
+
+
+@pytest.mark.parametrize("name", ALG_NAMES)
+@pytest.mark.parametrize("levels", [1, 2])
+def test_fmm_random_error(name, levels):
+    f = fmm(name)
+    alg = F.Alg([f] * levels)
+    m = n = k = 432 if name != "444" else 512
+    say("fmm rand", name, levels, m)
+    A, B, C = O.uniform(m, k, 0, 0), O.uniform(k, n, 0, 1), O.uniform(m, n, 0, 2)
+    ref = O.dgemm(1.0, A, B, 1.0, C.copy(order="F"))
+    got = run_fmm(alg, 1.0, A, B, 1.0, C)
+    err = O.normalized_error(got, ref, A, B)
+    say(f"  normalised error {err:.3e}")
+    assert err < TOL_FP64[levels]
+
+
+def test_beta_zero_ignores_nan():
+    m, n, k = 70, 90, 50
+    A, B = O.uniform(m, k, 2, 0), O.uniform(k, n, 2, 1)
+    C = np.full((m, n), np.nan, order="F")
+    got = run_fmm(F.Alg([F.strassen()]), 1.0, A, B, 0.0, C)
+    assert np.all(np.isfinite(got))
+    assert O.normalized_error(got, A @ B, A, B) < 1e-15
+
+
+def test_alpha_zero_and_k_zero_scale_c():
+    A, B, C = O.uniform(20, 0, 1, 0), O.uniform(0, 30, 1, 1), O.uniform(20, 30, 1, 2)
+    alg = F.Alg([F.strassen()])
+    dA, dB, dC = F.DeviceMatrix(20, 0), F.DeviceMatrix(0, 30), dev(C)
+    F.dgemm_fmm(alg, 20, 30, 0, 1.0, dA, dB, 3.0, dC)
+    assert np.array_equal(dC.to_numpy(), 3.0 * C)
+    A = O.uniform(20, 10, 1, 0)
+    B = O.uniform(10, 30, 1, 1)
+    got = run_fmm(alg, 0.0, A, B, -1.0, C)
+    assert np.array_equal(got, -C)
+
+
+@pytest.mark.parametrize("ta,tb,tc", [(True, False, False), (False, True, False), (True, True, True),
+                                      (False, False, True)])
+def test_operand_layouts(ta, tb, tc):
+    """Row-major (transposed) operands through the native strides."""
+    m, n, k = 150, 170, 130
+    say("layouts", ta, tb, tc)
+    A, B, C = O.integers(m, k, 4, 0, 3), O.integers(k, n, 4, 1, 3), O.integers(m, n, 4, 2, 3)
+    want = O.dgemm(1.0, A, B, 1.0, C.copy(order="F"))
+    def put(X, t):
+        # row-major storage = column-major storage of X^T
+        return dev(np.asfortranarray(X.T)) if t else dev(X)
+    dA, dB, dC = put(A, ta), put(B, tb), put(C, tc)
+    rsA, csA = (k, 1) if ta else (1, m)
+    rsB, csB = (n, 1) if tb else (1, k)
+    rsC, csC = (n, 1) if tc else (1, m)
+    alg = F.Alg([fmm("232")])
+    F.check(F.lib().fmmhip_dgemm_fmm(alg.h, 0, m, n, k, 1.0, dA.ptr, rsA, csA, dB.ptr, rsB, csB, 1.0,
+                                     dC.ptr, rsC, csC, None), "dgemm_fmm layouts")
+    got = dC.to_numpy()
+    if tc:
+        got = got.T
+    assert np.array_equal(got, want)
+
+
+def test_dropin_reference_case_host_objects():
+    """test/test_strassen_oapi.c:44-224 restated: 3 x 13 x 7, alpha = beta = 1,
+    column-stored objects in host memory, ||C - C_ref||_F printed."""
+    m, n, k = 3, 13, 7
+    A, B, C = O.uniform(m, k, 0, 0), O.uniform(k, n, 0, 1), O.uniform(m, n, 0, 2)
+    C_ref = O.dgemm_naive(A, B, C.copy(order="F"))
+    Cw = C.copy(order="F")
+    L = F.lib()
+    assert L.bli_plugin_register_fmm_blis() == F.BLIS_SUCCESS
+    one, s1 = F.obj_scalar(1.0)
+    L.bli_strassen_ab(one, F.obj_attach(A, m, k, 1, m), F.obj_attach(B, k, n, 1, k), one,
+                      F.obj_attach(Cw, m, n, 1, m))
+    resid = np.linalg.norm(Cw - C_ref)
+    say(f"{m:5d}\t {n:5d}\t {k:5d}\t resid {resid:.2g}")
+    assert resid < 1e-14
+
+
+@pytest.mark.parametrize("name", ["222", "232", "333"])
+def test_dropin_ex_device_objects_and_trans(name):
+    m, n, k = 201, 190, 180
+    A, B, C = O.integers(m, k, 6, 0, 3), O.integers(k, n, 6, 1, 3), O.integers(m, n, 6, 2, 3)
+    want = O.dgemm(2.0, A, B, -1.0, C.copy(order="F"))
+    dAt = dev(np.asfortranarray(A.T))          # A stored transposed, trans bit set
+    dB, dC = dev(B), dev(C)
+    oa = F.obj_attach(dAt.ptr.value, k, m, 1, k)
+    oa.info |= F.TRANS_BIT
+    ob = F.obj_attach(dB.ptr.value, k, n, 1, k)
+    oc = F.obj_attach(dC.ptr.value, m, n, 1, m)
+    al, s1 = F.obj_scalar(2.0)
+    be, s2 = F.obj_scalar(-1.0)
+    F.lib().bli_strassen_ab_ex(al, oa, ob, be, oc, fmm(name))
+    assert np.array_equal(dC.to_numpy(), want)
+
+
+def test_dropin_row_stored_host_and_levels():
+    m, n, k = 120, 100, 90
+    A, B, C = O.integers(m, k, 8, 0, 3), O.integers(k, n, 8, 1, 3), O.integers(m, n, 8, 2, 3)
+    want = O.dgemm(1.0, A, B, 1.0, C.copy(order="F"))
+    Ar, Br, Cr = np.ascontiguousarray(A), np.ascontiguousarray(B), np.ascontiguousarray(C)
+    one, s1 = F.obj_scalar(1.0)
+    levels = (F.FmmT * 2)(fmm("222"), fmm("322"))
+    F.lib().bli_fmm_ab_ex(one, F.obj_attach(Ar, m, k, k, 1), F.obj_attach(Br, k, n, n, 1), one,
+                          F.obj_attach(Cr, m, n, n, 1), levels, 2, F.LEAF_NATIVE)
+    assert np.array_equal(Cr, want)
+
+
+def test_classical_object_entry_float():
+    m, n, k = 64, 80, 96
+    A, B, C = O.uniform_f32(m, k, 1, 0), O.uniform_f32(k, n, 1, 1), O.uniform_f32(m, n, 1, 2)
+    Cw = C.copy(order="F")
+    one, s1 = F.obj_scalar(1.0)
+    F.lib().bli_gemm_hip(one, F.obj_attach(A, m, k, 1, m, F.BLIS_FLOAT), F.obj_attach(B, k, n, 1, k, F.BLIS_FLOAT),
+                         one, F.obj_attach(Cw, m, n, 1, m, F.BLIS_FLOAT))
+    ref = C.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)
+    err = O.normalized_error(Cw.astype(np.float64), ref, A.astype(np.float64), B.astype(np.float64))
+    assert err < 1e-7
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_fp32_leaf_and_sgemm_fmm(levels):
+    m = n = k = 512
+    A, B, C = O.uniform(m, k, 0, 0), O.uniform(k, n, 0, 1), O.uniform(m, n, 0, 2)
+    ref = O.dgemm(1.0, A, B, 1.0, C.copy(order="F"))
+    alg = F.Alg([F.strassen()] * levels)
+    got = run_fmm(alg, 1.0, A, B, 1.0, C, leaf=F.LEAF_F32)
+    err = O.normalized_error(got, ref, A, B)
+    say(f"fp32 leaf L{levels} err {err:.3e}")
+    assert err < 1e-7
+    As, Bs, Cs = A.astype(np.float32), B.astype(np.float32), C.astype(np.float32)
+    dA, dB, dC = dev(np.asfortranarray(As)), dev(np.asfortranarray(Bs)), dev(np.asfortranarray(Cs))
+    F.sgemm_fmm(alg, m, n, k, 1.0, dA, dB, 1.0, dC)
+    got = dC.to_numpy().astype(np.float64)
+    ref32 = Cs.astype(np.float64) + As.astype(np.float64) @ Bs.astype(np.float64)
+    assert O.normalized_error(got, ref32, As.astype(np.float64), Bs.astype(np.float64)) < 1e-7
+
+
+def test_large_integer_fmm_equals_classical_on_device():
+    """Size-independent property at 4096^3: with integer inputs both paths are
+    exact, so 2-level Strassen must equal the classical kernel bit for bit."""
+    n = 4096
+    say("large int", n)
+    A, B = F.DeviceMatrix(n, n).fill_int(9, 0, 2), F.DeviceMatrix(n, n).fill_int(9, 1, 2)
+    C1, C2 = F.DeviceMatrix(n, n).fill_int(9, 2, 2), F.DeviceMatrix(n, n).fill_int(9, 2, 2)
+    F.dgemm(n, n, n, 1.0, A, B, 1.0, C1)
+    F.dgemm_fmm(F.Alg([F.strassen()] * 2), n, n, n, 1.0, A, B, 1.0, C2)
+    F.sync()
+    assert F.fro_diff(C1, C2) == 0.0
+    assert F.fro(C1) > 0

@@ -1,0 +1,193 @@
+"""CPU tests of the library's host side through its C ABI: the library loads
+and exports what include/fmm_blis_amd.h declares, the coefficient reader
+(new_fmm, frame/base/bli_read_fmm.c:24-80) agrees with the oracle on every
+file and rejects malformed ones, STRASSEN_FMM equals 222.txt, composition
+and the Brent check, and the error behaviour of the object entries.  No
+device computation happens here."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+import textwrap
+
+import numpy as np
+import pytest
+
+import fmm_blis_amd as F
+import oracle as O
+from conftest import ALG_NAMES, ROOT, alg_path
+
+HAS_GPU = os.path.exists("/dev/kfd")
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "fmm_blis_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
+    keywords = {"if", "sizeof", "defined"}
+    return sorted(n for n in names if n not in keywords and not n.isupper())
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", F.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = {line.split()[-1] for line in out.stdout.splitlines() if line.strip()}
+    declared = header_functions() + ["STRASSEN_FMM", "FMM_BLIS_PACK_UKR", "FMM_BLIS_GEMM_UKR"]
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+    L = F.lib()
+    for n in F.EXPORTED:
+        getattr(L, n)
+
+
+@pytest.mark.parametrize("name", ALG_NAMES)
+def test_new_fmm_matches_oracle(name):
+    f = F.new_fmm(alg_path(name))
+    o = O.parse(alg_path(name))
+    try:
+        assert (f.m_tilde, f.k_tilde, f.n_tilde, f.R) == (o.mt, o.kt, o.nt, o.R)
+        for got, want in zip(f.tables(), o.tables()):
+            assert np.array_equal(np.array(got), want)
+    finally:
+        F.free_fmm(f)
+
+
+def test_strassen_builtin_equals_222():
+    s = F.strassen()
+    o = O.parse(alg_path("222"))
+    assert (s.m_tilde, s.k_tilde, s.n_tilde, s.R) == (2, 2, 2, 7)
+    for got, want in zip(s.tables(), o.tables()):
+        assert np.array_equal(np.array(got), want)
+
+
+BAD_FILES = {
+    "bad_header": "2 2 x 7\n",
+    "short_header": "2 2 2\n1 0\n",
+    "short_row": "1 1 1 2\n1\n#\n1 1\n#\n1 1\n",
+    "too_few_rows": "1 1 1 1\n1\n#\n1\n",
+    "too_many_rows": "1 1 1 1\n1\n1\n1\n1\n",
+    "token": "1 1 1 1\n1\n#\nq\n#\n1\n",
+    "zero_dim": "0 1 1 1\n",
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD_FILES))
+def test_new_fmm_rejects_malformed(tmp_path, case):
+    p = tmp_path / f"{case}.txt"
+    p.write_text(BAD_FILES[case])
+    f = F.new_fmm(str(p))
+    assert f.R == 0 and not f.U and not f.V and not f.W
+    F.free_fmm(f)
+
+
+def test_new_fmm_missing_file(tmp_path):
+    f = F.new_fmm(str(tmp_path / "nope.txt"))
+    assert f.R == 0 and not f.U
+
+
+def test_new_fmm_long_lines_and_comments(tmp_path):
+    # a 1x1x1 R=200 algorithm whose rows are far longer than 128 characters
+    # (the reference's LINE_N), with comments between the blocks
+    R = 200
+    u = [1] + [0] * (R - 1)
+    rows = [" ".join(map(str, u))] * 3
+    p = tmp_path / "long.txt"
+    p.write_text(f"1 1 1 {R}\n# comment\n{rows[0]}\n#\n{rows[1]}\n# another\n\n{rows[2]}\n")
+    f = F.new_fmm(str(p))
+    assert f.R == R and f.tables()[0][0][:2] == [1, 0]
+    a = F.Alg([f])
+    assert a.info()["R"] == 1          # all-zero products are dropped
+    assert a.brent_violations() == 0
+    F.free_fmm(f)
+
+
+@pytest.mark.parametrize("names", [("222",), ("232",), ("322",), ("333",), ("444",), ("222", "222"),
+                                   ("222", "333"), ("232", "322"), ("444", "444")])
+def test_alg_compose_matches_oracle(names):
+    fs = [F.new_fmm(alg_path(n)) for n in names]
+    a = F.Alg(fs)
+    info = a.info()
+    g = O.parse(alg_path(names[0]))
+    for n in names[1:]:
+        g = O.compose(g, O.parse(alg_path(n)))
+    U, V, W = g.tables()
+    assert (info["mt"], info["kt"], info["nt"], info["R"]) == (g.mt, g.kt, g.nt, g.R)
+    assert (info["nnz_u"], info["nnz_v"], info["nnz_w"]) == tuple(int(np.count_nonzero(x)) for x in (U, V, W))
+    if names != ("444", "444"):
+        assert a.brent_violations() == 0
+    a.close()
+    for f in fs:
+        F.free_fmm(f)
+
+
+def test_alg_create_rejects_invalid_table(tmp_path):
+    text = open(alg_path("222")).read().replace("1 0 1 0 1 -1 0", "1 0 1 0 1 1 0", 1)
+    p = tmp_path / "bad222.txt"
+    p.write_text(text)
+    f = F.new_fmm(str(p))
+    assert f.R == 7
+    with pytest.raises(F.FmmError):
+        F.Alg([f])
+    F.free_fmm(f)
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the behaviour without a device")
+def test_dropin_without_device_records_error():
+    L = F.lib()
+    assert L.bli_plugin_register_fmm_blis() == F.BLIS_FAILURE
+    L.fmm_blis_amd_set_abort_on_error(0)
+    try:
+        A = np.zeros((3, 7), order="F"); B = np.zeros((7, 13), order="F"); C = np.zeros((3, 13), order="F")
+        oa, ob, oc = F.obj_attach(A, 3, 7, 1, 3), F.obj_attach(B, 7, 13, 1, 7), F.obj_attach(C, 3, 13, 1, 3)
+        one, s1 = F.obj_scalar(1.0)
+        L.bli_strassen_ab(one, oa, ob, one, oc)
+        assert L.fmm_blis_amd_last_error() == 5   # FMMHIP_ERR_NODEVICE
+    finally:
+        L.fmm_blis_amd_set_abort_on_error(1)
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the behaviour without a device")
+def test_dropin_aborts_by_default_like_blis():
+    code = textwrap.dedent(f"""
+        import sys, numpy as np
+        sys.path.insert(0, {os.path.join(ROOT, 'fmm-blis_amd')!r})
+        import fmm_blis_amd as F
+        A = np.zeros((3, 7), order="F"); B = np.zeros((7, 13), order="F"); C = np.zeros((3, 13), order="F")
+        one, s = F.obj_scalar(1.0)
+        F.lib().bli_strassen_ab(one, F.obj_attach(A, 3, 7, 1, 3), F.obj_attach(B, 7, 13, 1, 7), one,
+                                F.obj_attach(C, 3, 13, 1, 3))
+        print("returned")
+    """)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "returned" not in r.stdout
+    assert "bli_strassen_ab_ex" in r.stderr
+
+
+def test_obj_attach_layout():
+    A = np.arange(12.0).reshape(3, 4, order="F")
+    o = F.obj_attach(A, 3, 4, 1, 3)
+    assert (o.dim[0], o.dim[1], o.rs, o.cs, o.elem_size, o.info & 7) == (3, 4, 1, 3, 8, F.BLIS_DOUBLE)
+    import ctypes
+    assert ctypes.sizeof(F.ObjT) == 160      # BLIS obj_t on LP64
+
+
+def test_c_header_layout_matches_binding(tmp_path):
+    """The C declarations of obj_t and fmm_t (compiled with gcc) have the
+    offsets the Python binding uses."""
+    src = tmp_path / "lay.c"
+    src.write_text(textwrap.dedent("""
+        #include <stdio.h>
+        #include <stddef.h>
+        #include "fmm_blis_amd.h"
+        int main(void) {
+          printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(obj_t), offsetof(obj_t, buffer),
+                 offsetof(obj_t, rs), offsetof(obj_t, scalar), offsetof(obj_t, n_panel),
+                 sizeof(fmm_t), offsetof(fmm_t, k_tilde), offsetof(fmm_t, U));
+          return 0;
+        }"""))
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
+    want = [ctypes.sizeof(F.ObjT), F.ObjT.buffer.offset, F.ObjT.rs.offset, F.ObjT.scalar.offset,
+            F.ObjT.n_panel.offset, ctypes.sizeof(F.FmmT), F.FmmT.k_tilde.offset, F.FmmT.U.offset]
+    assert got == want
