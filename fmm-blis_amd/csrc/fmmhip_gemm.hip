@@ -75,7 +75,13 @@ struct KArgs {
 
 // Leaf arithmetic traits.
 template <typename TL> struct Leaf;
+// Accumulator layout of the 16x16 output block, lane l, register q:
+//   FP64 (v_mfma_f64_16x16x4_f64):  row l/16 + 4q,    col l%16
+//   FP32 (v_mfma_f32_16x16x4_f32):  row 4(l/16) + q,  col l%16
+// (both checked on the device with known operands).
 template <> struct Leaf<double> {
+    static constexpr int ROW_LANE = 1;   // row step per lane group (l/16)
+    static constexpr int ROW_REG = 4;    // row step per register q
     typedef d4_t acc_t;
     static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c)
     {
@@ -83,6 +89,8 @@ template <> struct Leaf<double> {
     }
 };
 template <> struct Leaf<float> {
+    static constexpr int ROW_LANE = 4;
+    static constexpr int ROW_REG = 1;
     typedef f4_t acc_t;
     static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c)
     {
@@ -283,8 +291,8 @@ void fmm_gemm_kernel(KArgs a)
     stB.init(tid, ldb);
 
     // epilogue coordinates of this lane's accumulator element (i, j, q):
-    // row = m0 + wm + 16 i + 4 (lane/16) + q, col = n0 + wn + 16 j + lane%16
-    const int er0 = wm + 4 * (lane >> 4);
+    // row = m0 + wm + 16 i + ROW_LANE (lane/16) + ROW_REG q, col = n0 + wn + 16 j + lane%16
+    const int er0 = wm + Leaf<TL>::ROW_LANE * (lane >> 4);
     const int ec0 = wn + (lane & 15);
 
 
@@ -362,7 +370,7 @@ void fmm_gemm_kernel(KArgs a)
             const double beta = a.beta;
             const long rsC = C_COL ? 1 : a.rsC;
             const int rs32 = (int)rsC, cs32 = (int)a.csC;
-            // lane's element (i, j, q) is at tile row er0 + 16 i + q, col ec0 + 16 j
+            // lane's element (i, j, q) is at tile row er0 + 16 i + ROW_REG q, col ec0 + 16 j
             TO* ct = Cb + we.off + (long)(m0 + er0) * rsC + (long)(n0 + ec0) * a.csC;
             const int erows = we.rows - m0 - er0, ecols = we.cols - n0 - ec0;
 #pragma unroll
@@ -371,7 +379,7 @@ void fmm_gemm_kernel(KArgs a)
                 for (int i = 0; i < FI; i++) {
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        const int dr = 16 * i + q, dc = 16 * j;
+                        const int dr = 16 * i + Leaf<TL>::ROW_REG * q, dc = 16 * j;
                         if (c_in || (dr < erows && dc < ecols)) {
                             TO* p = ct + (C_COL ? dr + dc * cs32 : dr * rs32 + dc * cs32);
                             double old = 0.0;

@@ -92,9 +92,34 @@ extern "C" int fmmhip_malloc(void** p, size_t bytes)
     return FMMHIP_SUCCESS;
 }
 extern "C" int fmmhip_free(void* p) { INIT_OR_RETURN(); HIP_OK(hipFree(p)); return 0; }
-extern "C" int fmmhip_memcpy_h2d(void* d, const void* s, size_t b) { INIT_OR_RETURN(); HIP_OK(hipMemcpy(d, s, b, hipMemcpyHostToDevice)); return 0; }
-extern "C" int fmmhip_memcpy_d2h(void* d, const void* s, size_t b) { INIT_OR_RETURN(); HIP_OK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost)); return 0; }
-extern "C" int fmmhip_memset(void* p, int v, size_t b) { INIT_OR_RETURN(); HIP_OK(hipMemset(p, v, b)); return 0; }
+// The library's streams are non-blocking (they do not wait on the null
+// stream), and a pageable hipMemcpy may return before its DMA lands, so the
+// copies below synchronise the device on both sides: on return the data is
+// in place and visible to every stream.
+extern "C" int fmmhip_memcpy_h2d(void* d, const void* s, size_t b)
+{
+    INIT_OR_RETURN();
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(d, s, b, hipMemcpyHostToDevice));
+    HIP_OK(hipDeviceSynchronize());
+    return 0;
+}
+extern "C" int fmmhip_memcpy_d2h(void* d, const void* s, size_t b)
+{
+    INIT_OR_RETURN();
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost));
+    HIP_OK(hipDeviceSynchronize());
+    return 0;
+}
+extern "C" int fmmhip_memset(void* p, int v, size_t b)
+{
+    INIT_OR_RETURN();
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemset(p, v, b));
+    HIP_OK(hipDeviceSynchronize());
+    return 0;
+}
 extern "C" int fmmhip_stream_create(void** s)
 {
     INIT_OR_RETURN();
