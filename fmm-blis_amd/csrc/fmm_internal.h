@@ -38,6 +38,7 @@ struct fmmhip_alg_s {
     int *w_ptr, *w_part, *w_coef, *w_first;
     long nnz_u, nnz_v, nnz_w;
     int  max_nnz_u, max_nnz_v, max_nnz_w;
+    int  shape;                 /* kernel launch shape, -1 = chosen per call */
     void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
 };
 

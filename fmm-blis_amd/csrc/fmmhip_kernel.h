@@ -1,0 +1,511 @@
+// fmmhip_kernel.h -- the fused FMM kernel for gfx950 (device templates),
+// included by the per-datatype translation units fmmhip_gemm_*.hip.
+//
+// One kernel computes C := beta*C + alpha * sum_r W_r (x) (S_r * T_r) with
+//   S_r = sum U[(i,p)][r] A_ip,  T_r = sum V[(p,j)][r] B_pj
+// for any composed <m~,k~,n~;R> algorithm; the classical GEMM is the
+// 1x1x1, R = 1 algorithm through the same code.
+//
+// Reference counterparts (dnparikh/fmm-blis):
+//   pack stage   plugin/ref_kernels/bli_packm_fmm_ref.c:89-147 -- the
+//                weighted sum of the nonzero parts is formed in registers
+//                while the tile streams from HBM and is written to LDS once
+//   leaf         the native BLIS micro-kernel called at
+//                plugin/ref_kernels/bli_gemm_fmm_ref.c:85-97 -- FP64 MFMA
+//                (v_mfma_f64_16x16x4_f64) out of LDS
+//   W epilogue   plugin/ref_kernels/bli_gemm_fmm_ref.c:101-120 -- each
+//                accumulator tile is added into every C part with a nonzero
+//                W coefficient, clipped to the part; no M_r is stored
+//   R loop       frame/base/bli_strassen_ab.c:248-294 -- inside the kernel
+//
+// Work decomposition: one workgroup per BM x BN tile position of the leaf,
+// looping over all R products.  It is the only writer of that tile position
+// in every C part, so the read-modify-write of C is race-free without
+// atomics and deterministic.  The first product that writes a part applies
+// beta (beta == 0 never reads C).
+#ifndef FMMHIP_KERNEL_H
+#define FMMHIP_KERNEL_H
+
+#include <hip/hip_runtime.h>
+
+#include "fmm_internal.h"
+
+namespace fmmk {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+
+constexpr int BM = 128;          // tile rows (leaf M)
+constexpr int BN = 128;          // tile cols (leaf N)
+constexpr int BK = 16;           // k per LDS stage
+// LDS tiles are kept [BK][LDS_ROW] (mn innermost): rows of 16 lanes read 128
+// contiguous bytes and the +16 pad puts consecutive k rows in opposite bank
+// halves.  (A k-major layout for k-contiguous operands removes the 8-way
+// conflicts of their transposing store but costs two registers, which
+// spills at two workgroups per CU; measured no faster.)
+constexpr int LDS_ROW = BM + 16;
+constexpr int LDS_TILE = BK * LDS_ROW;   // elements per tile buffer
+
+// Element (mn, k) of an LDS tile.
+__device__ __forceinline__ constexpr int lds_at(int mn, int k) { return k * LDS_ROW + mn; }
+
+// Launch shape: NT threads as a WAVES_M x WAVES_N grid of waves, OCC waves
+// per SIMD.
+// MULTI: the parts of a product after the first are interleaved with the
+// MFMA sub-blocks (one extra register set in flight); otherwise they are
+// loaded after the MFMA block.
+//   Shape 0: 512 threads, 64 x 32 per wave, two workgroups per CU, single
+//   Shape 1: 256 threads, 64 x 64 per wave, two workgroups per CU, multi
+template <int SHAPE> struct Cfg;
+template <> struct Cfg<0> { static constexpr int NT = 512, WAVES_M = 2, WAVES_N = 4, OCC = 4; static constexpr bool MULTI = false; };
+template <> struct Cfg<1> { static constexpr int NT = 256, WAVES_M = 2, WAVES_N = 2, OCC = 2; static constexpr bool MULTI = true; };
+
+template <int SHAPE> struct Geo {
+    static constexpr int NT = Cfg<SHAPE>::NT;
+    static constexpr int WTM = BM / Cfg<SHAPE>::WAVES_M;
+    static constexpr int WTN = BN / Cfg<SHAPE>::WAVES_N;
+    static constexpr int FI = WTM / 16;            // 16x16 MFMA blocks per wave along M
+    static constexpr int FJ = WTN / 16;            // ... along N
+    static constexpr int PAIRS = BM * BK / 2 / NT; // staged pairs per thread per operand
+    static constexpr int MN_PAIRS = BM / 2;        // pairs along mn in one k row
+    static constexpr int K_ROWS = NT / MN_PAIRS;   // k rows per pass (mn-contiguous)
+    static constexpr int K_PAIRS = BK / 2;         // pairs along k in one mn column
+    static constexpr int MN_COLS = NT / K_PAIRS;   // mn columns per pass (k-contiguous)
+    static_assert(PAIRS * NT * 2 == BM * BK, "staging covers the tile");
+};
+
+struct KArgs {
+    const void* A;
+    const void* B;
+    void* C;
+    long lda;                    // the non-unit stride of A (A_MN: A(i,p) at A[i + p*lda])
+    long ldb;                    // the non-unit stride of B (B_MN: B(p,j) at B[p*ldb + j])
+    long rsC, csC;
+    int lm, lk, ln;              // leaf dims
+    int min_m, min_k, min_n;     // smallest part extents: tiles inside them need no bounds checks
+    int R;
+    int tiles_m, tiles_n;
+    int va, vb;                  // pairs of A / B are 16-byte aligned (vector loads allowed)
+    const int* ptr;              // u, v, w CSR pointers, 3*(R+1)
+    const fmm_entry_t* ent;
+    double alpha, beta;
+};
+
+// Accumulator layout of the 16x16 output block, lane l, register q:
+//   FP64 (v_mfma_f64_16x16x4_f64):  row l/16 + 4q,    col l%16
+//   FP32 (v_mfma_f32_16x16x4_f32):  row 4(l/16) + q,  col l%16
+// (both checked on the device with known operands).  Operands: lane l holds
+// A[l%16][l/16] and B[l/16][l%16] of the 16x4 / 4x16 blocks.
+template <typename TL> struct Leaf;
+template <> struct Leaf<double> {
+    static constexpr int ROW_LANE = 1;   // row step per lane group (l/16)
+    static constexpr int ROW_REG = 4;    // row step per register q
+    typedef d4_t acc_t;
+    static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Leaf<float> {
+    static constexpr int ROW_LANE = 4;
+    static constexpr int ROW_REG = 1;
+    typedef f4_t acc_t;
+    static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename TI> struct Pair;
+template <> struct Pair<double> { typedef d2_t t; };
+template <> struct Pair<float> { typedef f2_t t; };
+
+// Staging of one operand tile (BM or BN along "mn", BK along k) for one
+// k-step: the weighted sum over the product's nonzero parts.  The raw values
+// of up to NE parts are loaded before the MFMA block of the current k-step
+// (their loads are in flight while the matrix cores run) and combined after
+// it; parts beyond NE are loaded and accumulated after the block.
+//   MN_CONTIG: the operand is contiguous along mn (pairs along mn),
+//              otherwise along k (pairs along k).
+//   IS_B:      the entry's (rows, cols) are (k, mn) rather than (mn, k).
+// Addresses are a uniform tile pointer (scalar registers) plus a per-thread
+// 32-bit offset fixed for the whole kernel.
+template <typename TI, bool MN_CONTIG, bool IS_B, int SHAPE>
+struct Stage {
+    static constexpr int PAIRS = Geo<SHAPE>::PAIRS, MN_PAIRS = Geo<SHAPE>::MN_PAIRS, K_ROWS = Geo<SHAPE>::K_ROWS,
+                         K_PAIRS = Geo<SHAPE>::K_PAIRS, MN_COLS = Geo<SHAPE>::MN_COLS;
+    double sum[PAIRS][2];        // the weighted sum being formed
+    double tmp[PAIRS][2];        // raw values of the part in flight
+    int lmn0, lk0;               // tile-local coordinates of the thread's pair 0
+    int off0;                    // its element offset in the tile
+
+    // pair s sits at (lmn0, lk0 + K_ROWS*s) when contiguous along mn,
+    // at (lmn0 + MN_COLS*s, lk0) when contiguous along k
+    static constexpr int dmn(int s) { return MN_CONTIG ? 0 : MN_COLS * s; }
+    static constexpr int dk(int s) { return MN_CONTIG ? K_ROWS * s : 0; }
+
+    __device__ __forceinline__ void init(int tid, long ld)
+    {
+        if (MN_CONTIG) {
+            lmn0 = 2 * (tid % MN_PAIRS);
+            lk0 = tid / MN_PAIRS;
+            off0 = lmn0 + lk0 * (int)ld;
+        } else {
+            lk0 = 2 * (tid % K_PAIRS);
+            lmn0 = tid / K_PAIRS;
+            off0 = lmn0 * (int)ld + lk0;
+        }
+    }
+
+    // Raw values of part e over the tile at (mn0, k0); zero outside the
+    // part's extent.  `interior`: the whole tile lies inside every part.
+    template <bool VEC>
+    __device__ __forceinline__ void load(const TI* __restrict__ base, const fmm_entry_t& e, long ld,
+                                         int mn0, int k0, bool interior, double (&x)[PAIRS][2]) const
+    {
+        const TI* t = base + e.off + (MN_CONTIG ? (long)mn0 + (long)k0 * ld : (long)mn0 * ld + k0) + off0;
+        typedef typename Pair<TI>::t pair_t;
+        const int ld32 = (int)ld;
+        if (interior) {
+#pragma unroll
+            for (int s = 0; s < PAIRS; s++) {
+                const TI* p = t + (MN_CONTIG ? dk(s) * ld32 : dmn(s) * ld32);
+                if (VEC) {
+                    pair_t q = *(const pair_t*)p;
+                    x[s][0] = (double)q[0];
+                    x[s][1] = (double)q[1];
+                } else {
+                    x[s][0] = (double)p[0];
+                    x[s][1] = (double)p[1];
+                }
+            }
+            return;
+        }
+        const int emn = (IS_B ? e.cols : e.rows) - mn0;   // valid extent in tile coords
+        const int ek = (IS_B ? e.rows : e.cols) - k0;
+#pragma unroll
+        for (int s = 0; s < PAIRS; s++) {
+            const TI* p = t + (MN_CONTIG ? dk(s) * ld32 : dmn(s) * ld32);
+            const int mn_0 = lmn0 + dmn(s), k_0 = lk0 + dk(s);
+            // element 1 is one step along the contiguous dimension
+            const bool ok0 = mn_0 < emn && k_0 < ek;
+            const bool ok1 = MN_CONTIG ? (mn_0 + 1 < emn && k_0 < ek) : (mn_0 < emn && k_0 + 1 < ek);
+            double x0 = 0.0, x1 = 0.0;
+            if (VEC && ok1) {
+                pair_t q = *(const pair_t*)p;
+                x0 = (double)q[0];
+                x1 = (double)q[1];
+            } else {
+                if (ok0) x0 = (double)p[0];
+                if (ok1) x1 = (double)p[1];
+            }
+            x[s][0] = x0;
+            x[s][1] = x1;
+        }
+    }
+
+    // sum := raw part e (scaled later by scale_first)
+    template <bool VEC>
+    __device__ __forceinline__ void load_first(const TI* base, const fmm_entry_t& e, long ld, int mn0, int k0,
+                                               bool interior)
+    {
+        load<VEC>(base, e, ld, mn0, k0, interior, sum);
+    }
+
+    __device__ __forceinline__ void scale_first(double c)
+    {
+        if (c != 1.0) {
+#pragma unroll
+            for (int s = 0; s < PAIRS; s++) { sum[s][0] *= c; sum[s][1] *= c; }
+        }
+    }
+
+    // tmp := raw part e (in flight until add_tmp)
+    template <bool VEC>
+    __device__ __forceinline__ void load_tmp(const TI* base, const fmm_entry_t& e, long ld, int mn0, int k0,
+                                             bool interior)
+    {
+        load<VEC>(base, e, ld, mn0, k0, interior, tmp);
+    }
+
+    __device__ __forceinline__ void add_tmp(double c)
+    {
+#pragma unroll
+        for (int s = 0; s < PAIRS; s++) {
+            sum[s][0] = fma(c, tmp[s][0], sum[s][0]);
+            sum[s][1] = fma(c, tmp[s][1], sum[s][1]);
+        }
+    }
+
+    // Writes the staged sums into an LDS tile (layout lds_at).
+    template <typename TL>
+    __device__ __forceinline__ void store(TL* lds) const
+    {
+        TL* q = lds + lds_at(lmn0, lk0);
+#pragma unroll
+        for (int s = 0; s < PAIRS; s++) {
+            TL* qs = q + lds_at(dmn(s), dk(s));
+            qs[0] = (TL)sum[s][0];
+            qs[lds_at(MN_CONTIG ? 1 : 0, MN_CONTIG ? 0 : 1)] = (TL)sum[s][1];   // next along the contiguous dim
+        }
+    }
+};
+
+// XCD-aware tile order: consecutive workgroup ids are dealt round-robin to
+// the 8 XCDs, so ids are first regrouped so that each XCD receives a
+// contiguous run of tiles, then the run is walked in groups of 8 tile rows
+// (column-major inside a group) so that concurrently running workgroups on
+// one XCD share A row panels and B column panels in that XCD's L2.
+__device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& tm, int& tn)
+{
+    const int nwg = tiles_m * tiles_n;
+    int lin = bid;
+    if ((nwg & 7) == 0) lin = (bid & 7) * (nwg >> 3) + (bid >> 3);
+    const int GROUP = 8;
+    const int per_group = GROUP * tiles_n;
+    const int g = lin / per_group;
+    const int first_m = g * GROUP;
+    const int gsz = min(tiles_m - first_m, GROUP);
+    const int in = lin - g * per_group;
+    tm = first_m + in % gsz;
+    tn = in / gsz;
+}
+
+template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
+__global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>::NT),
+                          amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
+void fmm_gemm_kernel(KArgs a)
+{
+    constexpr int WAVES_M = Cfg<SHAPE>::WAVES_M;
+    constexpr bool MULTI = Cfg<SHAPE>::MULTI;
+    constexpr int WTM = Geo<SHAPE>::WTM, WTN = Geo<SHAPE>::WTN, FI = Geo<SHAPE>::FI, FJ = Geo<SHAPE>::FJ;
+    typedef typename Leaf<TL>::acc_t acc_t;
+    __shared__ TL sA[2][LDS_TILE];
+    __shared__ TL sB[2][LDS_TILE];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave % WAVES_M) * WTM;
+    const int wn = (wave / WAVES_M) * WTN;
+
+    int tm, tn;
+    tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    const TI* __restrict__ Ab = (const TI*)a.A;
+    const TI* __restrict__ Bb = (const TI*)a.B;
+    TO* __restrict__ Cb = (TO*)a.C;
+    const int R = a.R;
+    const int* __restrict__ uptr = a.ptr;
+    const int* __restrict__ vptr = a.ptr + (R + 1);
+    const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
+    const fmm_entry_t* __restrict__ ent = a.ent;
+    const int nk = (a.lk + BK - 1) / BK;
+    const long lda = a.lda, ldb = a.ldb;
+
+    // tiles inside every part need no bounds checks
+    const bool a_rows_in = m0 + BM <= a.min_m;
+    const bool b_cols_in = n0 + BN <= a.min_n;
+    const bool c_in = a_rows_in && b_cols_in;
+
+    Stage<TI, A_MN, false, SHAPE> stA;
+    Stage<TI, B_MN, true, SHAPE> stB;
+    stA.init(tid, lda);
+    stB.init(tid, ldb);
+
+    // epilogue coordinates of this lane's accumulator element (i, j, q):
+    // row = m0 + wm + 16 i + ROW_LANE (lane/16) + ROW_REG q, col = n0 + wn + 16 j + lane%16
+    const int er0 = wm + Leaf<TL>::ROW_LANE * (lane >> 4);
+    const int ec0 = wn + (lane & 15);
+
+    for (int r = 0; r < R; r++) {
+        const int u0 = uptr[r], nu = uptr[r + 1] - u0;
+        const int v0 = vptr[r], nv = vptr[r + 1] - v0;
+        const int ng = nu > nv ? nu : nv;   // load groups: part g of A and part g of B
+        // the product's first parts, held for all k-steps
+        const fmm_entry_t ea0 = ent[u0], eb0 = ent[v0];
+
+        acc_t acc[FI][FJ];
+#pragma unroll
+        for (int i = 0; i < FI; i++)
+#pragma unroll
+            for (int j = 0; j < FJ; j++) acc[i][j] = (acc_t){0, 0, 0, 0};
+
+        // prologue: stage k-step 0 (all groups, one after another)
+        {
+            const bool ia = a_rows_in && BK <= a.min_k, ib = b_cols_in && BK <= a.min_k;
+            stA.template load_first<VEC>(Ab, ea0, lda, m0, 0, ia);
+            stB.template load_first<VEC>(Bb, eb0, ldb, n0, 0, ib);
+            stA.scale_first(ea0.coef);
+            stB.scale_first(eb0.coef);
+            for (int g = 1; g < nu; g++) {
+                stA.template load_tmp<VEC>(Ab, ent[u0 + g], lda, m0, 0, ia);
+                stA.add_tmp(ent[u0 + g].coef);
+            }
+            for (int g = 1; g < nv; g++) {
+                stB.template load_tmp<VEC>(Bb, ent[v0 + g], ldb, n0, 0, ib);
+                stB.add_tmp(ent[v0 + g].coef);
+            }
+            stA.template store<TL>(sA[0]);
+            stB.template store<TL>(sB[0]);
+        }
+        __syncthreads();
+
+        // Main loop.  While the MFMAs of k-step kt run out of LDS buffer
+        // kt&1, the next k-step's parts stream in: group 0 and 1 are issued
+        // before MFMA sub-block 0; after sub-block g the group g+1 that
+        // arrived is folded into the sum and group g+2 is issued, so each
+        // part's load is covered by a quarter of the MFMA block.
+        constexpr int KS = BK / 4;
+        for (int kt = 0; kt < nk; kt++) {
+            const int cur = kt & 1;
+            const bool more = kt + 1 < nk;
+            const int k0 = (kt + 1) * BK;
+            const bool kin = k0 + BK <= a.min_k;
+            const bool ia = a_rows_in && kin, ib = b_cols_in && kin;
+            if (more) {
+                stA.template load_first<VEC>(Ab, ea0, lda, m0, k0, ia);
+                stB.template load_first<VEC>(Bb, eb0, ldb, n0, k0, ib);
+                if (MULTI) {
+                    if (1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + 1], lda, m0, k0, ia);
+                    if (1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + 1], ldb, n0, k0, ib);
+                }
+            }
+            const TL* la = sA[cur];
+            const TL* lb = sB[cur];
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                const int kk = 4 * ks + (lane >> 4);
+                TL fa[FI], fb[FJ];
+#pragma unroll
+                for (int i = 0; i < FI; i++) fa[i] = la[lds_at(wm + 16 * i + (lane & 15), kk)];
+#pragma unroll
+                for (int j = 0; j < FJ; j++) fb[j] = lb[lds_at(wn + 16 * j + (lane & 15), kk)];
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) acc[i][j] = Leaf<TL>::mfma(fa[i], fb[j], acc[i][j]);
+                if (MULTI && more) {
+                    if (ks == 0) {
+                        stA.scale_first(ea0.coef);
+                        stB.scale_first(eb0.coef);
+                    }
+                    const int g = ks + 1;            // group that was in flight
+                    if (g < ng) {
+                        if (g < nu) stA.add_tmp(ent[u0 + g].coef);
+                        if (g < nv) stB.add_tmp(ent[v0 + g].coef);
+                        if (g + 1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + g + 1], lda, m0, k0, ia);
+                        if (g + 1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + g + 1], ldb, n0, k0, ib);
+                    }
+                }
+            }
+            if (more) {
+                if (MULTI) {
+                    // groups past the MFMA block: the one in flight, then serially
+                    for (int g = KS + 1; g < ng; g++) {
+                        if (g < nu) stA.add_tmp(ent[u0 + g].coef);
+                        if (g < nv) stB.add_tmp(ent[v0 + g].coef);
+                        if (g + 1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + g + 1], lda, m0, k0, ia);
+                        if (g + 1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + g + 1], ldb, n0, k0, ib);
+                    }
+                } else {
+                    // single: every further part after the MFMA block, one
+                    // at a time (fewest registers)
+                    stA.scale_first(ea0.coef);
+                    stB.scale_first(eb0.coef);
+                    for (int g = 1; g < nu; g++) {
+                        stA.template load_tmp<VEC>(Ab, ent[u0 + g], lda, m0, k0, ia);
+                        stA.add_tmp(ent[u0 + g].coef);
+                    }
+                    for (int g = 1; g < nv; g++) {
+                        stB.template load_tmp<VEC>(Bb, ent[v0 + g], ldb, n0, k0, ib);
+                        stB.add_tmp(ent[v0 + g].coef);
+                    }
+                }
+                stA.template store<TL>(sA[cur ^ 1]);
+                stB.template store<TL>(sB[cur ^ 1]);
+            }
+            __syncthreads();
+        }
+
+        // W epilogue: C_part += alpha*w*acc for every nonzero w of product r;
+        // the part's first writer applies beta (beta == 0: C is not read).
+        const int w0 = wptr[r], w1 = wptr[r + 1];
+        for (int e = w0; e < w1; e++) {
+            const fmm_entry_t we = ent[e];
+            const double lam = a.alpha * we.coef;
+            const bool first = we.first != 0;
+            const double beta = a.beta;
+            const long rsC = C_COL ? 1 : a.rsC;
+            const int rs32 = (int)rsC, cs32 = (int)a.csC;
+            // lane's element (i, j, q) is at tile row er0 + 16 i + ROW_REG q, col ec0 + 16 j
+            TO* ct = Cb + we.off + (long)(m0 + er0) * rsC + (long)(n0 + ec0) * a.csC;
+            const int erows = we.rows - m0 - er0, ecols = we.cols - n0 - ec0;
+#pragma unroll
+            for (int j = 0; j < FJ; j++) {
+#pragma unroll
+                for (int i = 0; i < FI; i++) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int dr = 16 * i + Leaf<TL>::ROW_REG * q, dc = 16 * j;
+                        if (c_in || (dr < erows && dc < ecols)) {
+                            TO* p = ct + (C_COL ? dr + dc * cs32 : dr * rs32 + dc * cs32);
+                            double old = 0.0;
+                            if (!first) old = (double)*p;
+                            else if (beta != 0.0) old = beta * (double)*p;
+                            *p = (TO)fma(lam, (double)acc[i][j][q], old);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Turns runtime layout flags into template arguments.
+template <typename TI, typename TL, typename TO, int SHAPE, bool VEC>
+int launch_layouts(const KArgs& ka, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st)
+{
+#define FMMK_GO(AM, BMN, CC)                                                                           \
+    hipLaunchKernelGGL((fmm_gemm_kernel<TI, TL, TO, SHAPE, VEC, AM, BMN, CC>), dim3(nwg),                \
+                       dim3(Cfg<SHAPE>::NT), 0, st, ka)
+    const int sel = (a_mn ? 4 : 0) | (b_mn ? 2 : 0) | (c_col ? 1 : 0);
+    switch (sel) {
+    case 0: FMMK_GO(false, false, false); break;
+    case 1: FMMK_GO(false, false, true); break;
+    case 2: FMMK_GO(false, true, false); break;
+    case 3: FMMK_GO(false, true, true); break;
+    case 4: FMMK_GO(true, false, false); break;
+    case 5: FMMK_GO(true, false, true); break;
+    case 6: FMMK_GO(true, true, false); break;
+    default: FMMK_GO(true, true, true); break;
+    }
+#undef FMMK_GO
+    return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+}
+
+// Vector loads need both operands' pairs 16-byte aligned (ka.va && ka.vb).
+template <typename TI, typename TL, typename TO>
+int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st)
+{
+    const bool vec = ka.va && ka.vb;
+    if (shape == 1) {
+        if (vec) return launch_layouts<TI, TL, TO, 1, true>(ka, a_mn, b_mn, c_col, nwg, st);
+        return launch_layouts<TI, TL, TO, 1, false>(ka, a_mn, b_mn, c_col, nwg, st);
+    }
+    if (vec) return launch_layouts<TI, TL, TO, 0, true>(ka, a_mn, b_mn, c_col, nwg, st);
+    return launch_layouts<TI, TL, TO, 0, false>(ka, a_mn, b_mn, c_col, nwg, st);
+}
+
+}  // namespace fmmk
+
+// Per-datatype launchers (one translation unit each).
+int fmmk_launch_d64(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
+int fmmk_launch_d_f32leaf(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg,
+                          hipStream_t st);
+int fmmk_launch_s32(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
+
+#endif

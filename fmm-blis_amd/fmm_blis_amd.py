@@ -99,6 +99,7 @@ _SIGS = {
     "fmmhip_alg_destroy": (None, [_P]),
     "fmmhip_alg_info": (_I, [_P] + [ctypes.POINTER(_I)] * 4 + [ctypes.POINTER(_L)] * 3),
     "fmmhip_alg_brent_violations": (_L, [_P]),
+    "fmmhip_alg_set_shape": (_I, [_P, _I]),
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
@@ -183,6 +184,9 @@ class Alg:
 
     def brent_violations(self):
         return lib().fmmhip_alg_brent_violations(self.h)
+
+    def set_shape(self, shape):
+        check(lib().fmmhip_alg_set_shape(self.h, shape), "alg_set_shape")
 
     def close(self):
         if self.h:

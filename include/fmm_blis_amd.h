@@ -211,6 +211,10 @@ int  fmmhip_alg_info(fmmhip_alg_t alg, int* mt, int* kt, int* nt, int* R,
                      long* nnz_u, long* nnz_v, long* nnz_w);
 /* Number of equations of the Brent system the tables violate (0 = exact). */
 long fmmhip_alg_brent_violations(fmmhip_alg_t alg);
+/* Kernel launch shape for this algorithm: 0 = 512 threads (one part per
+ * product loaded ahead), 1 = 256 threads (two parts ahead), -1 = chosen per
+ * call (default).  Used by the autotuner. */
+int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
 
 /* C := beta*C + alpha*A*B on device pointers, asynchronous on `stream`
  * (NULL = the library's stream).  Element (i,j) of X is X[i*rsX + j*csX];
