@@ -39,6 +39,7 @@ struct fmmhip_alg_s {
     long nnz_u, nnz_v, nnz_w;
     int  max_nnz_u, max_nnz_v, max_nnz_w;
     int  shape;                 /* kernel launch shape, -1 = chosen per call */
+    int  pack;                  /* 0 fused, 1 pre-summed workspace, -1 = chosen per call */
     void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
 };
 
@@ -70,6 +71,11 @@ int fmmhip_scale_s(float* C, long m, long n, long rsC, long csC, float beta, voi
 int fmmhip_pointer_is_device(const void* p);
 /* The library's default stream (created on first use). */
 void* fmmhip_default_stream(void);
+/* Summation stage of the pre-summed pack mode (fmmhip_presum.hip). */
+int fmmhip_presum_d(const double* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,
+                    int Lc, int Ls, int ldo, double* S, void* stream);
+int fmmhip_presum_s(const float* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,
+                    int Lc, int Ls, int ldo, float* S, void* stream);
 
 #ifdef __cplusplus
 }

@@ -18,13 +18,22 @@ namespace {
 
 // ---- host side -------------------------------------------------------------
 
-struct DevDesc {
-    fmm_geom_t geo;
+// Device copy of an entry list: [ptr (3*(R+1) int32), pad, entries].
+struct DevList {
     int32_t* ptr_h = nullptr;
     fmm_entry_t* ent_h = nullptr;
     long nent = 0;
-    void* dev = nullptr;         // [ptr (3*(R+1) int32), pad, entries]
+    void* dev = nullptr;
     size_t ent_off = 0;
+    const int* dptr() const { return (const int*)dev; }
+    const fmm_entry_t* dent() const { return (const fmm_entry_t*)((char*)dev + ent_off); }
+};
+
+struct DevDesc {
+    fmm_geom_t geo;
+    DevList fused;               // every nonzero part of every product
+    DevList presum;              // one part per product (the workspace leaf), W as in fused
+    bool has_presum = false;
     unsigned long last_use = 0;
 };
 
@@ -42,18 +51,97 @@ bool same_geo(const fmm_geom_t& a, const fmm_geom_t& b)
            a.rsB == b.rsB && a.csB == b.csB && a.rsC == b.rsC && a.csC == b.csC;
 }
 
+void free_list(DevList& l)
+{
+    if (l.dev) (void)hipFree(l.dev);
+    free(l.ptr_h);
+    free(l.ent_h);
+    l = DevList();
+}
+
 void free_desc(DevDesc* d)
 {
-    if (d->dev) (void)hipFree(d->dev);
-    free(d->ptr_h);
-    free(d->ent_h);
+    free_list(d->fused);
+    free_list(d->presum);
     delete d;
 }
 
-// Returns the device copy of the algorithm's entries for geometry g,
-// uploading (stream-ordered) when it is not cached.
-int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t stream, const int** dptr,
-             const fmm_entry_t** dent)
+int upload_list(DevList& l, int R, hipStream_t stream)
+{
+    size_t ptr_bytes = sizeof(int32_t) * 3 * (size_t)(R + 1);
+    l.ent_off = (ptr_bytes + 255) & ~(size_t)255;
+    size_t total = l.ent_off + sizeof(fmm_entry_t) * (size_t)(l.nent + 1);
+    if (hipMalloc(&l.dev, total) != hipSuccess) { (void)hipGetLastError(); l.dev = nullptr; return FMMHIP_ERR_NOMEM; }
+    if (hipMemcpyAsync(l.dev, l.ptr_h, ptr_bytes, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync((char*)l.dev + l.ent_off, l.ent_h, sizeof(fmm_entry_t) * (size_t)l.nent,
+                       hipMemcpyHostToDevice, stream) != hipSuccess)
+        return FMMHIP_ERR_HIP;
+    return FMMHIP_SUCCESS;
+}
+
+// Workspace leaf layout of the pre-summed mode for one operand: R leaves of
+// Lc x Ls (contiguous dim c), leading dimension ldo = Lc rounded up to even.
+struct WsLayout {
+    int Lc, Ls, ldo;
+    size_t elems(int R) const { return (size_t)ldo * Ls * R; }
+};
+
+WsLayout ws_layout_a(const fmm_geom_t& g, bool a_mn)
+{
+    WsLayout w;
+    w.Lc = (int)(a_mn ? g.lm : g.lk);
+    w.Ls = (int)(a_mn ? g.lk : g.lm);
+    w.ldo = (w.Lc + 1) & ~1;
+    return w;
+}
+
+WsLayout ws_layout_b(const fmm_geom_t& g, bool b_mn)
+{
+    WsLayout w;
+    w.Lc = (int)(b_mn ? g.ln : g.lk);
+    w.Ls = (int)(b_mn ? g.lk : g.ln);
+    w.ldo = (w.Lc + 1) & ~1;
+    return w;
+}
+
+// The pre-summed list: product r reads leaf r of the A and B workspaces
+// with coefficient 1 over the full leaf; its W entries are the fused ones.
+int build_presum_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const DevList& fused, bool a_mn,
+                      bool b_mn, DevList& out)
+{
+    const int R = alg->R;
+    const WsLayout wa = ws_layout_a(g, a_mn), wb = ws_layout_b(g, b_mn);
+    out.nent = 2L * R + alg->nnz_w;
+    out.ptr_h = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
+    out.ent_h = (fmm_entry_t*)calloc((size_t)out.nent + 1, sizeof(fmm_entry_t));
+    if (!out.ptr_h || !out.ent_h) return FMMHIP_ERR_NOMEM;
+    for (int r = 0; r <= R; r++) {
+        out.ptr_h[r] = r;
+        out.ptr_h[(R + 1) + r] = R + r;
+        out.ptr_h[2 * (R + 1) + r] = 2 * R + (fused.ptr_h[2 * (R + 1) + r] - (int32_t)(alg->nnz_u + alg->nnz_v));
+    }
+    for (int r = 0; r < R; r++) {
+        fmm_entry_t& ea = out.ent_h[r];
+        ea.off = (int64_t)r * wa.ldo * wa.Ls;
+        ea.rows = (int32_t)g.lm;
+        ea.cols = (int32_t)g.lk;
+        ea.coef = 1.0;
+        ea.part = -1;
+        fmm_entry_t& eb = out.ent_h[R + r];
+        eb.off = (int64_t)r * wb.ldo * wb.Ls;
+        eb.rows = (int32_t)g.lk;
+        eb.cols = (int32_t)g.ln;
+        eb.coef = 1.0;
+        eb.part = -1;
+    }
+    memcpy(out.ent_h + 2 * R, fused.ent_h + alg->nnz_u + alg->nnz_v, sizeof(fmm_entry_t) * (size_t)alg->nnz_w);
+    return FMMHIP_SUCCESS;
+}
+
+// Returns the cached descriptor for geometry g (building and uploading the
+// fused list, and the pre-summed list when asked, stream-ordered).
+int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool a_mn, bool b_mn, hipStream_t stream,
+             DevDesc** out)
 {
     if (!alg->dev_cache) {
         static std::mutex create_mu;
@@ -63,42 +151,65 @@ int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t stream, const in
     DevCache* cache = (DevCache*)alg->dev_cache;
     std::lock_guard<std::mutex> lk(cache->mu);
     cache->clock++;
-    for (DevDesc* d : cache->items) {
-        if (same_geo(d->geo, g)) {
-            d->last_use = cache->clock;
-            *dptr = (const int*)d->dev;
-            *dent = (const fmm_entry_t*)((char*)d->dev + d->ent_off);
-            return FMMHIP_SUCCESS;
+    DevDesc* d = nullptr;
+    for (DevDesc* c : cache->items)
+        if (same_geo(c->geo, g)) { d = c; break; }
+    if (!d) {
+        if (cache->items.size() >= kCacheCap) {
+            // evict the least recently used descriptor; kernels may still read it
+            size_t victim = 0;
+            for (size_t i = 1; i < cache->items.size(); i++)
+                if (cache->items[i]->last_use < cache->items[victim]->last_use) victim = i;
+            if (hipDeviceSynchronize() != hipSuccess) return FMMHIP_ERR_HIP;
+            free_desc(cache->items[victim]);
+            cache->items.erase(cache->items.begin() + victim);
         }
+        d = new DevDesc();
+        d->geo = g;
+        int rc = fmm_build_entries(alg, &g, &d->fused.ptr_h, &d->fused.ent_h, &d->fused.nent);
+        if (!rc) rc = upload_list(d->fused, alg->R, stream);
+        if (rc) { free_desc(d); return rc; }
+        cache->items.push_back(d);
     }
-    if (cache->items.size() >= kCacheCap) {
-        // evict the least recently used descriptor; kernels may still read it
-        size_t victim = 0;
-        for (size_t i = 1; i < cache->items.size(); i++)
-            if (cache->items[i]->last_use < cache->items[victim]->last_use) victim = i;
-        if (hipDeviceSynchronize() != hipSuccess) return FMMHIP_ERR_HIP;
-        free_desc(cache->items[victim]);
-        cache->items.erase(cache->items.begin() + victim);
-    }
-    DevDesc* d = new DevDesc();
-    d->geo = g;
-    int rc = fmm_build_entries(alg, &g, &d->ptr_h, &d->ent_h, &d->nent);
-    if (rc) { delete d; return rc; }
-    size_t ptr_bytes = sizeof(int32_t) * 3 * (size_t)(alg->R + 1);
-    d->ent_off = (ptr_bytes + 255) & ~(size_t)255;
-    size_t total = d->ent_off + sizeof(fmm_entry_t) * (size_t)(d->nent + 1);
-    if (hipMalloc(&d->dev, total) != hipSuccess) { (void)hipGetLastError(); d->dev = nullptr; free_desc(d); return FMMHIP_ERR_NOMEM; }
-    if (hipMemcpyAsync(d->dev, d->ptr_h, ptr_bytes, hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipMemcpyAsync((char*)d->dev + d->ent_off, d->ent_h, sizeof(fmm_entry_t) * (size_t)d->nent,
-                       hipMemcpyHostToDevice, stream) != hipSuccess) {
-        free_desc(d);
-        return FMMHIP_ERR_HIP;
+    if (want_presum && !d->has_presum) {
+        int rc = build_presum_list(alg, g, d->fused, a_mn, b_mn, d->presum);
+        if (!rc) rc = upload_list(d->presum, alg->R, stream);
+        if (rc) { free_list(d->presum); return rc; }
+        d->has_presum = true;
     }
     d->last_use = cache->clock;
-    cache->items.push_back(d);
-    *dptr = (const int*)d->dev;
-    *dent = (const fmm_entry_t*)((char*)d->dev + d->ent_off);
+    *out = d;
     return FMMHIP_SUCCESS;
+}
+
+// Per-stream workspace for the pre-summed mode (grow-only; a stream's
+// kernels are ordered, so one buffer per stream is race-free).
+struct Workspace {
+    hipStream_t stream;
+    void* p;
+    size_t cap;
+};
+std::mutex g_ws_mu;
+std::vector<Workspace> g_ws;
+
+void* workspace_get(hipStream_t st, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    for (Workspace& w : g_ws) {
+        if (w.stream != st) continue;
+        if (w.cap >= bytes) return w.p;
+        if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
+        (void)hipFree(w.p);
+        w.p = nullptr;
+        w.cap = 0;
+        if (hipMalloc(&w.p, bytes) != hipSuccess) { (void)hipGetLastError(); w.p = nullptr; return nullptr; }
+        w.cap = bytes;
+        return w.p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    g_ws.push_back(Workspace{st, p, bytes});
+    return p;
 }
 
 // Pairs along the contiguous dimension can be loaded as one 16-byte vector
@@ -115,6 +226,9 @@ bool pairs_aligned(const void* base, size_t esz, long ld, const fmm_entry_t* ent
 }
 
 enum class Kind { D64, D_F32LEAF, S32 };
+
+// Largest workspace the automatic choice takes for the pre-summed mode.
+constexpr double kWsFractionOfFree = 0.5;
 
 int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha,
                 const void* A, long rsA, long csA, const void* B, long rsB, long csB, double beta,
@@ -149,46 +263,81 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
     g.lk = (k + alg->kt - 1) / alg->kt;
     g.ln = (n + alg->nt - 1) / alg->nt;
     g.rsA = rsA; g.csA = csA; g.rsB = rsB; g.csB = csB; g.rsC = rsC; g.csC = csC;
+    const size_t esz = kind == Kind::S32 ? 4 : 8;
+    const bool multi = alg->max_nnz_u > 1 || alg->max_nnz_v > 1;
 
-    const int* dptr = nullptr;
-    const fmm_entry_t* dent = nullptr;
-    rc = get_desc(alg, g, st, &dptr, &dent);
-    if (rc) return rc;
-
-    // vector-load eligibility from the host copy of the entries
-    DevCache* cache = (DevCache*)alg->dev_cache;
-    bool va = false, vb = false;
-    // launch shape: the algorithm's setting, else one part in flight for
-    // classical-like algorithms and two for those with multi-part products
-    int shape = alg->shape;
-    if (shape < 0) shape = (alg->max_nnz_u > 1 || alg->max_nnz_v > 1) ? 1 : 0;
-    {
-        std::lock_guard<std::mutex> lk(cache->mu);
-        for (DevDesc* d : cache->items) {
-            if (!same_geo(d->geo, g)) continue;
-            size_t esz = kind == Kind::S32 ? 4 : 8;
-            va = pairs_aligned(A, esz, a_mn ? csA : rsA, d->ent_h, 0, alg->nnz_u);
-            vb = pairs_aligned(B, esz, b_mn ? rsB : csB, d->ent_h, alg->nnz_u, alg->nnz_u + alg->nnz_v);
-            break;
+    // pack mode: pre-summed when asked, or (automatic) for multi-part
+    // algorithms when the workspace fits in half of the free device memory
+    const WsLayout wa = ws_layout_a(g, a_mn), wb = ws_layout_b(g, b_mn);
+    const size_t ws_bytes = esz * (wa.elems(alg->R) + wb.elems(alg->R) + 64);
+    bool presum = alg->pack == 1;
+    if (alg->pack < 0 && multi) {
+        size_t fr = 0, tot = 0;
+        presum = hipMemGetInfo(&fr, &tot) == hipSuccess && ws_bytes < kWsFractionOfFree * (double)fr;
+    }
+    void* ws = nullptr;
+    if (presum) {
+        ws = workspace_get(st, ws_bytes);
+        if (!ws) {
+            if (alg->pack == 1) return FMMHIP_ERR_NOMEM;
+            presum = false;          // automatic choice falls back to the fused mode
         }
     }
 
+    DevDesc* d = nullptr;
+    rc = get_desc(alg, g, presum, a_mn, b_mn, st, &d);
+    if (rc) return rc;
+
     KArgs ka;
-    ka.A = A; ka.B = B; ka.C = C;
-    ka.lda = a_mn ? csA : rsA;
-    ka.ldb = b_mn ? rsB : csB;
+    ka.C = C;
     ka.rsC = rsC; ka.csC = csC;
     ka.lm = (int)g.lm; ka.lk = (int)g.lk; ka.ln = (int)g.ln;
-    ka.min_m = (int)(m / alg->mt); ka.min_k = (int)(k / alg->kt); ka.min_n = (int)(n / alg->nt);
+    ka.min_cm = (int)(m / alg->mt); ka.min_cn = (int)(n / alg->nt);
     ka.R = alg->R;
     ka.tiles_m = (int)((g.lm + BM - 1) / BM);
     ka.tiles_n = (int)((g.ln + BN - 1) / BN);
-    ka.ptr = dptr;
-    ka.ent = dent;
     ka.alpha = alpha;
     ka.beta = beta;
-    ka.va = va;
-    ka.vb = vb;
+    int shape = alg->shape;
+    if (presum) {
+        // summation stage: S_r and T_r for every product into the workspace
+        char* S = (char*)ws;
+        char* T = S + esz * wa.elems(alg->R);
+        T = (char*)(((uintptr_t)T + 255) & ~(uintptr_t)255);
+        const int R = alg->R;
+        if (kind == Kind::S32) {
+            rc = fmmhip_presum_s((const float*)A, a_mn ? csA : rsA, d->fused.dptr(), d->fused.dent(), R,
+                                 a_mn ? 0 : 1, wa.Lc, wa.Ls, wa.ldo, (float*)S, st);
+            if (!rc) rc = fmmhip_presum_s((const float*)B, b_mn ? rsB : csB, d->fused.dptr() + (R + 1),
+                                          d->fused.dent(), R, b_mn ? 1 : 0, wb.Lc, wb.Ls, wb.ldo, (float*)T, st);
+        } else {
+            rc = fmmhip_presum_d((const double*)A, a_mn ? csA : rsA, d->fused.dptr(), d->fused.dent(), R,
+                                 a_mn ? 0 : 1, wa.Lc, wa.Ls, wa.ldo, (double*)S, st);
+            if (!rc) rc = fmmhip_presum_d((const double*)B, b_mn ? rsB : csB, d->fused.dptr() + (R + 1),
+                                          d->fused.dent(), R, b_mn ? 1 : 0, wb.Lc, wb.Ls, wb.ldo, (double*)T, st);
+        }
+        if (rc) return rc;
+        ka.A = S;
+        ka.B = T;
+        ka.lda = wa.ldo;
+        ka.ldb = wb.ldo;
+        ka.min_m = (int)g.lm; ka.min_k = (int)g.lk; ka.min_n = (int)g.ln;   // full leaves
+        ka.ptr = d->presum.dptr();
+        ka.ent = d->presum.dent();
+        ka.va = ka.vb = 1;           // workspace leaves are 16-byte aligned with even ldo
+        if (shape < 0) shape = 0;    // one part per product
+    } else {
+        ka.A = A;
+        ka.B = B;
+        ka.lda = a_mn ? csA : rsA;
+        ka.ldb = b_mn ? rsB : csB;
+        ka.min_m = (int)(m / alg->mt); ka.min_k = (int)(k / alg->kt); ka.min_n = (int)(n / alg->nt);
+        ka.ptr = d->fused.dptr();
+        ka.ent = d->fused.dent();
+        ka.va = pairs_aligned(A, esz, ka.lda, d->fused.ent_h, 0, alg->nnz_u);
+        ka.vb = pairs_aligned(B, esz, ka.ldb, d->fused.ent_h, alg->nnz_u, alg->nnz_u + alg->nnz_v);
+        if (shape < 0) shape = multi ? 1 : 0;
+    }
     long nwg = (long)ka.tiles_m * ka.tiles_n;
     if (nwg <= 0) return FMMHIP_SUCCESS;
     if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
@@ -216,6 +365,24 @@ extern "C" int fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape)
 {
     if (!alg || shape < -1 || shape > 1) return FMMHIP_ERR_INVALID;
     alg->shape = shape;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack)
+{
+    if (!alg || pack < -1 || pack > 1) return FMMHIP_ERR_INVALID;
+    alg->pack = pack;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_workspace_release(void)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (hipDeviceSynchronize() != hipSuccess) return FMMHIP_ERR_HIP;
+    for (Workspace& w : g_ws) (void)hipFree(w.p);
+    g_ws.clear();
     return FMMHIP_SUCCESS;
 }
 

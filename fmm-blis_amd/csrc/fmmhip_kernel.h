@@ -84,7 +84,8 @@ struct KArgs {
     long ldb;                    // the non-unit stride of B (B_MN: B(p,j) at B[p*ldb + j])
     long rsC, csC;
     int lm, lk, ln;              // leaf dims
-    int min_m, min_k, min_n;     // smallest part extents: tiles inside them need no bounds checks
+    int min_m, min_k, min_n;     // smallest A-rows / k / B-cols part extents: tiles inside need no checks
+    int min_cm, min_cn;          // smallest C part extents (epilogue)
     int R;
     int tiles_m, tiles_n;
     int va, vb;                  // pairs of A / B are 16-byte aligned (vector loads allowed)
@@ -309,7 +310,7 @@ void fmm_gemm_kernel(KArgs a)
     // tiles inside every part need no bounds checks
     const bool a_rows_in = m0 + BM <= a.min_m;
     const bool b_cols_in = n0 + BN <= a.min_n;
-    const bool c_in = a_rows_in && b_cols_in;
+    const bool c_in = m0 + BM <= a.min_cm && n0 + BN <= a.min_cn;
 
     Stage<TI, A_MN, false, SHAPE> stA;
     Stage<TI, B_MN, true, SHAPE> stB;

@@ -100,6 +100,8 @@ _SIGS = {
     "fmmhip_alg_info": (_I, [_P] + [ctypes.POINTER(_I)] * 4 + [ctypes.POINTER(_L)] * 3),
     "fmmhip_alg_brent_violations": (_L, [_P]),
     "fmmhip_alg_set_shape": (_I, [_P, _I]),
+    "fmmhip_alg_set_pack": (_I, [_P, _I]),
+    "fmmhip_workspace_release": (_I, []),
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
@@ -187,6 +189,9 @@ class Alg:
 
     def set_shape(self, shape):
         check(lib().fmmhip_alg_set_shape(self.h, shape), "alg_set_shape")
+
+    def set_pack(self, pack):
+        check(lib().fmmhip_alg_set_pack(self.h, pack), "alg_set_pack")
 
     def close(self):
         if self.h:

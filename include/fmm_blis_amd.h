@@ -215,6 +215,16 @@ long fmmhip_alg_brent_violations(fmmhip_alg_t alg);
  * product loaded ahead), 1 = 256 threads (two parts ahead), -1 = chosen per
  * call (default).  Used by the autotuner. */
 int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
+/* How the U/V-weighted operand sums are formed: 0 = fused (streamed from the
+ * operands into LDS inside the MFMA kernel, no workspace), 1 = pre-summed
+ * (an HBM-bandwidth pass writes every S_r and T_r once to a per-stream
+ * device workspace of R*(leaf(A) + leaf(B)) elements; the MFMA kernel then
+ * reads one part per product), -1 = chosen per call (default: pre-summed
+ * for multi-part algorithms when the workspace fits in half of the free
+ * device memory).  The W scatter is fused into the kernel in both modes. */
+int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
+/* Frees the pre-summed mode's workspaces (synchronises the device). */
+int  fmmhip_workspace_release(void);
 
 /* C := beta*C + alpha*A*B on device pointers, asynchronous on `stream`
  * (NULL = the library's stream).  Element (i,j) of X is X[i*rsX + j*csX];

@@ -108,11 +108,13 @@ FMM_SHAPES = [(3, 13, 7), (31, 33, 35), (64, 64, 64), (97, 101, 103), (258, 260,
 
 @pytest.mark.parametrize("name", ALG_NAMES)
 @pytest.mark.parametrize("levels", [1, 2])
-def test_fmm_exact_integers(name, levels):
+@pytest.mark.parametrize("pack", [0, 1])
+def test_fmm_exact_integers(name, levels, pack):
     """Bit-exact agreement with the oracle's FMM on integer inputs (uneven
-    splits included)."""
+    splits included), in the fused and the pre-summed pack modes."""
     f = fmm(name)
     alg = F.Alg([f] * levels)
+    alg.set_pack(pack)
     og = O.parse(alg_path(name))
     og = og if levels == 1 else O.compose(og, og)
     shapes = FMM_SHAPES if not (name == "444" and levels == 2) else [(20, 18, 17), (130, 140, 150)]
@@ -127,9 +129,11 @@ def test_fmm_exact_integers(name, levels):
 
 @pytest.mark.parametrize("name", ALG_NAMES)
 @pytest.mark.parametrize("levels", [1, 2])
-def test_fmm_random_error(name, levels):
+@pytest.mark.parametrize("pack", [0, 1])
+def test_fmm_random_error(name, levels, pack):
     f = fmm(name)
     alg = F.Alg([f] * levels)
+    alg.set_pack(pack)
     m = n = k = 432 if name != "444" else 512
     say("fmm rand", name, levels, m)
     A, B, C = O.uniform(m, k, 0, 0), O.uniform(k, n, 0, 1), O.uniform(m, n, 0, 2)
@@ -172,17 +176,19 @@ def test_operand_layouts(ta, tb, tc):
     def put(X, t):
         # row-major storage = column-major storage of X^T
         return dev(np.asfortranarray(X.T)) if t else dev(X)
-    dA, dB, dC = put(A, ta), put(B, tb), put(C, tc)
     rsA, csA = (k, 1) if ta else (1, m)
     rsB, csB = (n, 1) if tb else (1, k)
     rsC, csC = (n, 1) if tc else (1, m)
     alg = F.Alg([fmm("232")])
-    F.check(F.lib().fmmhip_dgemm_fmm(alg.h, 0, m, n, k, 1.0, dA.ptr, rsA, csA, dB.ptr, rsB, csB, 1.0,
-                                     dC.ptr, rsC, csC, None), "dgemm_fmm layouts")
-    got = dC.to_numpy()
-    if tc:
-        got = got.T
-    assert np.array_equal(got, want)
+    for pack in (0, 1):
+        dA, dB, dC = put(A, ta), put(B, tb), put(C, tc)
+        alg.set_pack(pack)
+        F.check(F.lib().fmmhip_dgemm_fmm(alg.h, 0, m, n, k, 1.0, dA.ptr, rsA, csA, dB.ptr, rsB, csB, 1.0,
+                                         dC.ptr, rsC, csC, None), "dgemm_fmm layouts")
+        got = dC.to_numpy()
+        if tc:
+            got = got.T
+        assert np.array_equal(got, want), pack
 
 
 def test_dropin_reference_case_host_objects():
