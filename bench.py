@@ -4,11 +4,14 @@
 
 One step = one C := A*B + C product through the C ABI (fmmhip_dgemm_fmm) on
 device-resident, seeded uniform[-1,1] operands (column-major).  W untimed
-warm-up steps, then K timed steps between device synchronisations; every
-launch is also bracketed by HIP events on the library's stream to give the
-kernel's average duration (roofline) and its latency percentiles.
+warm-up steps, then K timed steps between device synchronisations.  Every
+step is bracketed by HIP events on the library's stream (latency
+percentiles), and the library's launch timing records events around the
+summation stage and the MFMA kernel inside each step (roofline).
 
-Prints ONE JSON line (rank 0, N = 1).
+Prints ONE JSON line.  --profile runs one warm-up and one step of the same
+workload plus a 1 GiB device copy (the calibration kernel for the HBM byte
+counters) and nothing else, for rocprofv3.
 """
 import argparse
 import json
@@ -21,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "fmm-blis_amd"))
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 matrix peak
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def percentile(xs, q):
@@ -32,17 +36,35 @@ def percentile(xs, q):
 
 
 def leaf_flops(info, m, n, k):
+    """Executed leaf flops 2 R^L ceil(m/m~^L) ceil(k/k~^L) ceil(n/n~^L)."""
     lm = -(-m // info["mt"])
     lk = -(-k // info["kt"])
     ln = -(-n // info["nt"])
     return 2.0 * info["R"] * lm * lk * ln
 
 
+def read_traffic(tag):
+    """HBM bytes per launch of the MFMA kernel for this workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by
+    the same counters on a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{tag}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(seconds_target=15.0):
     """The oracle's 1-level Strassen (222) at 2048^3 on the host cores: the
     CPU port of the reference's path (the reference itself is not built)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle as O
     n = 2048
     A, B, C = O.uniform(n, n, 0, 0), O.uniform(n, n, 0, 1), O.uniform(n, n, 0, 2)
@@ -70,8 +92,11 @@ def main():
     ap.add_argument("--alg", default="222")
     ap.add_argument("--levels", type=int, default=2)
     ap.add_argument("--leaf", default="f64", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--pack", type=int, default=-1, help="0 fused, 1 pre-summed, -1 automatic")
     ap.add_argument("--classical-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against the classical kernel")
+    ap.add_argument("--profile", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU sharding is a "
@@ -87,22 +112,35 @@ def main():
     else:
         f = F.new_fmm(os.path.join(ROOT, "tests", "golden", "algorithms", f"{args.alg}.txt"))
     alg = F.Alg([f] * args.levels)
+    alg.set_pack(args.pack)
     info = alg.info()
     leaf = {"f64": F.LEAF_NATIVE, "f32": F.LEAF_F32, "bf16x3": F.LEAF_BF16X3}[args.leaf]
+    tag = f"{args.alg}L{args.levels}_{args.leaf}_{n}"
 
     A = F.DeviceMatrix(m, k).fill_uniform(0, 0)
     B = F.DeviceMatrix(k, n).fill_uniform(0, 1)
     C = F.DeviceMatrix(m, n).fill_uniform(0, 2)
-    C0 = F.DeviceMatrix(m, n).fill_uniform(0, 2)
     F.sync()
 
     def step():
         F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, C, leaf=leaf)
 
+    if args.profile:
+        step()
+        step()
+        gib = 1 << 30
+        src, dst = F.DeviceBuffer(gib), F.DeviceBuffer(gib)
+        F.check(F.lib().fmmhip_memcpy_d2d(dst.ptr, src.ptr, gib, None), "copy")
+        F.sync()
+        print(json.dumps({"profile": tag, "steps": 2, "calibration_copy_bytes": gib}), flush=True)
+        return
+
     for _ in range(args.warmup):
         step()
     F.sync()
     evs = [(F.Event(), F.Event()) for _ in range(args.steps)]
+    F.timing_read()                       # drop any earlier record
+    F.timing_enable(True)
     F.sync()
     t0 = time.perf_counter()
     for e0, e1 in evs:
@@ -111,35 +149,44 @@ def main():
         e1.record()
     F.sync()
     wall = time.perf_counter() - t0
+    F.timing_enable(False)
+    launches, kern_total, presum_total, kern_max = F.timing_read()
     lat = [e0.elapsed_ms(e1) for e0, e1 in evs]
     ms_step = wall * 1e3 / args.steps
-    kern_ms = sum(lat) / len(lat)
+    kern_ms = kern_total / max(launches, 1)
+    presum_ms = presum_total / max(launches, 1)
     eff = 2.0 * m * n * k / (ms_step * 1e-3) / 1e9
 
     # classical FP64 MFMA GEMM on the same operands (its own C)
-    Cc = F.DeviceMatrix(m, n).fill_uniform(0, 2)
-    F.dgemm(m, n, k, 1.0, A, B, 1.0, Cc)
-    F.sync()
-    ce = [(F.Event(), F.Event()) for _ in range(args.classical_steps)]
-    for e0, e1 in ce:
-        e0.record()
+    cls_gflops = None
+    if args.classical_steps > 0:
+        Cc = F.DeviceMatrix(m, n).fill_uniform(0, 2)
         F.dgemm(m, n, k, 1.0, A, B, 1.0, Cc)
-        e1.record()
-    F.sync()
-    cls_ms = sum(e0.elapsed_ms(e1) for e0, e1 in ce) / len(ce)
-    cls_gflops = 2.0 * m * n * k / (cls_ms * 1e-3) / 1e9
+        F.sync()
+        ce = [(F.Event(), F.Event()) for _ in range(args.classical_steps)]
+        for e0, e1 in ce:
+            e0.record()
+            F.dgemm(m, n, k, 1.0, A, B, 1.0, Cc)
+            e1.record()
+        F.sync()
+        cls_ms = sum(e0.elapsed_ms(e1) for e0, e1 in ce) / len(ce)
+        cls_gflops = 2.0 * m * n * k / (cls_ms * 1e-3) / 1e9
+        Cc.free()
 
-    # accuracy: one fresh FMM and one fresh classical product from the same C0
-    Cf = F.DeviceMatrix(m, n).fill_uniform(0, 2)
-    Cr = F.DeviceMatrix(m, n).fill_uniform(0, 2)
-    F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, Cf, leaf=leaf)
-    F.dgemm(m, n, k, 1.0, A, B, 1.0, Cr)
-    F.sync()
-    rel = F.fro_diff(Cf, Cr) / (F.fro(A) * F.fro(B))
+    # accuracy: one fresh FMM and one fresh classical product from the same C
+    rel = None
+    if not args.no_check:
+        Cf = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+        Cr = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+        F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, Cf, leaf=leaf)
+        F.dgemm(m, n, k, 1.0, A, B, 1.0, Cr)
+        F.sync()
+        rel = F.fro_diff(Cf, Cr) / (F.fro(A) * F.fro(B))
 
     fl = leaf_flops(info, m, n, k)
-    peak = FP64_PEAK_TFLOPS if args.leaf == "f64" else FP32_PEAK_TFLOPS
+    peak = {"f64": FP64_PEAK_TFLOPS, "f32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS}[args.leaf]
     achieved = fl / (kern_ms * 1e-3) / 1e12
+    pack = "pre-summed" if presum_ms > 1e-3 else "fused"
     out = {
         "metric": "effective FP64 GFLOPS (2mnk/t) FMM vs classical at 16384^3; p99 GEMM latency",
         "value": round(eff, 1),
@@ -156,18 +203,19 @@ def main():
         "config": {
             "workload": f"{args.levels}-level <{f.m_tilde},{f.k_tilde},{f.n_tilde};{f.R}> FMM, FP64 C += A*B, "
                         f"m=n=k={n}, column-major",
-            "m": m, "n": n, "k": k, "alg": args.alg, "levels": args.levels, "leaf": args.leaf,
+            "m": m, "n": n, "k": k, "alg": args.alg, "levels": args.levels, "leaf": args.leaf, "pack": pack,
             "composed": [info["mt"], info["kt"], info["nt"], info["R"]],
-            "classical_fp64_gflops": round(cls_gflops, 1),
-            "fmm_over_classical": round(eff / cls_gflops, 4),
+            "classical_fp64_gflops": round(cls_gflops, 1) if cls_gflops else None,
+            "fmm_over_classical": round(eff / cls_gflops, 4) if cls_gflops else None,
             "rel_err_vs_classical": rel,
+            "summation_ms_per_step": round(presum_ms, 3),
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3),
-                       "mean": round(kern_ms, 3)},
+                       "mean": round(sum(lat) / len(lat), 3)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "fmm_gemm_kernel", "leaf_flops_per_launch": fl,
-                     "effective_over_peak": round(eff / 1e3 / peak, 4)},
+                     "frac": round(achieved / peak, 4), "traffic": read_traffic(tag),
+                     "kernel": "fmm_gemm_kernel", "kernel_ms": round(kern_ms, 3),
+                     "leaf_flops_per_launch": fl, "effective_over_peak": round(eff / 1e3 / peak, 4)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

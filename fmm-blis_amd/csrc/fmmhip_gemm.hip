@@ -227,6 +227,22 @@ bool pairs_aligned(const void* base, size_t esz, long ld, const fmm_entry_t* ent
 
 enum class Kind { D64, D_F32LEAF, S32 };
 
+// Optional per-launch timing (fmmhip_timing_*): HIP events recorded on the
+// launch stream around the summation stage and around the MFMA kernel.
+struct TimedCall {
+    hipEvent_t e0, e1, e2;      // before summation, before kernel, after kernel
+    bool presum;
+};
+std::mutex g_tm_mu;
+bool g_tm_on = false;
+std::vector<TimedCall> g_tm;
+
+void tm_record(hipEvent_t* ev, hipStream_t st)
+{
+    if (hipEventCreate(ev) == hipSuccess) (void)hipEventRecord(*ev, st);
+    else *ev = nullptr;
+}
+
 // Largest workspace the automatic choice takes for the pre-summed mode.
 constexpr double kWsFractionOfFree = 0.5;
 
@@ -299,6 +315,13 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
     ka.alpha = alpha;
     ka.beta = beta;
     int shape = alg->shape;
+    bool timed;
+    {
+        std::lock_guard<std::mutex> lk(g_tm_mu);
+        timed = g_tm_on;
+    }
+    TimedCall tc{nullptr, nullptr, nullptr, presum};
+    if (timed) tm_record(&tc.e0, st);
     if (presum) {
         // summation stage: S_r and T_r for every product into the workspace
         char* S = (char*)ws;
@@ -343,12 +366,18 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
     if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
 
     const bool c_col = rsC == 1;
+    if (timed) tm_record(&tc.e1, st);
     switch (kind) {
-    case Kind::D64: return fmmk_launch_d64(ka, shape, a_mn, b_mn, c_col, (int)nwg, st);
-    case Kind::D_F32LEAF: return fmmk_launch_d_f32leaf(ka, shape, a_mn, b_mn, c_col, (int)nwg, st);
-    case Kind::S32: return fmmk_launch_s32(ka, shape, a_mn, b_mn, c_col, (int)nwg, st);
+    case Kind::D64: rc = fmmk_launch_d64(ka, shape, a_mn, b_mn, c_col, (int)nwg, st); break;
+    case Kind::D_F32LEAF: rc = fmmk_launch_d_f32leaf(ka, shape, a_mn, b_mn, c_col, (int)nwg, st); break;
+    case Kind::S32: rc = fmmk_launch_s32(ka, shape, a_mn, b_mn, c_col, (int)nwg, st); break;
     }
-    return FMMHIP_ERR_INVALID;
+    if (timed) {
+        tm_record(&tc.e2, st);
+        std::lock_guard<std::mutex> lk(g_tm_mu);
+        g_tm.push_back(tc);
+    }
+    return rc;
 }
 
 fmmhip_alg_t classical_alg()
@@ -372,6 +401,43 @@ extern "C" int fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack)
 {
     if (!alg || pack < -1 || pack > 1) return FMMHIP_ERR_INVALID;
     alg->pack = pack;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_timing_enable(int on)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    g_tm_on = on != 0;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_timing_read(long* launches, double* kernel_ms, double* presum_ms, double* max_kernel_ms)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    long n = 0;
+    double k_ms = 0.0, p_ms = 0.0, k_max = 0.0;
+    for (TimedCall& c : g_tm) {
+        float a = 0.f, b = 0.f;
+        if (c.e0 && c.e1 && c.e2 && hipEventSynchronize(c.e2) == hipSuccess &&
+            hipEventElapsedTime(&a, c.e0, c.e1) == hipSuccess && hipEventElapsedTime(&b, c.e1, c.e2) == hipSuccess) {
+            n++;
+            p_ms += a;
+            k_ms += b;
+            if (b > k_max) k_max = b;
+        }
+        if (c.e0) (void)hipEventDestroy(c.e0);
+        if (c.e1) (void)hipEventDestroy(c.e1);
+        if (c.e2) (void)hipEventDestroy(c.e2);
+    }
+    g_tm.clear();
+    if (launches) *launches = n;
+    if (kernel_ms) *kernel_ms = k_ms;
+    if (presum_ms) *presum_ms = p_ms;
+    if (max_kernel_ms) *max_kernel_ms = k_max;
     return FMMHIP_SUCCESS;
 }
 

@@ -112,6 +112,12 @@ extern "C" int fmmhip_memcpy_d2h(void* d, const void* s, size_t b)
     HIP_OK(hipDeviceSynchronize());
     return 0;
 }
+extern "C" int fmmhip_memcpy_d2d(void* d, const void* s, size_t b, void* stream)
+{
+    INIT_OR_RETURN();
+    HIP_OK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, S(stream)));
+    return 0;
+}
 extern "C" int fmmhip_memset(void* p, int v, size_t b)
 {
     INIT_OR_RETURN();

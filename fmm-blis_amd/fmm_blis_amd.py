@@ -102,6 +102,9 @@ _SIGS = {
     "fmmhip_alg_set_shape": (_I, [_P, _I]),
     "fmmhip_alg_set_pack": (_I, [_P, _I]),
     "fmmhip_workspace_release": (_I, []),
+    "fmmhip_timing_enable": (_I, [_I]),
+    "fmmhip_timing_read": (_I, [ctypes.POINTER(_L)] + [ctypes.POINTER(_D)] * 3),
+    "fmmhip_memcpy_d2d": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
@@ -317,6 +320,18 @@ def fro(X):
 
 def sync():
     check(lib().fmmhip_device_sync(), "device_sync")
+
+
+def timing_enable(on=True):
+    check(lib().fmmhip_timing_enable(1 if on else 0), "timing_enable")
+
+
+def timing_read():
+    """(launches, kernel_ms_total, presum_ms_total, kernel_ms_max) since the last read."""
+    n, k, p, mx = _L(), _D(), _D(), _D()
+    check(lib().fmmhip_timing_read(ctypes.byref(n), ctypes.byref(k), ctypes.byref(p), ctypes.byref(mx)),
+          "timing_read")
+    return n.value, k.value, p.value, mx.value
 
 
 class Event:

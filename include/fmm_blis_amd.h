@@ -226,6 +226,13 @@ int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
 /* Frees the pre-summed mode's workspaces (synchronises the device). */
 int  fmmhip_workspace_release(void);
 
+/* Launch timing: while enabled, every FMM/GEMM call records HIP events on
+ * its stream around the summation stage and the MFMA kernel.  _read waits
+ * for them, returns the number of timed calls, the summed kernel and
+ * summation times and the longest kernel (ms), and clears the record. */
+int fmmhip_timing_enable(int on);
+int fmmhip_timing_read(long* launches, double* kernel_ms, double* presum_ms, double* max_kernel_ms);
+
 /* C := beta*C + alpha*A*B on device pointers, asynchronous on `stream`
  * (NULL = the library's stream).  Element (i,j) of X is X[i*rsX + j*csX];
  * for A and B one of rs, cs must be 1.  The leaf `leaf` applies to the
@@ -256,6 +263,8 @@ int fmmhip_free(void* p);
 int fmmhip_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int fmmhip_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int fmmhip_memset(void* p, int value, size_t bytes);
+/* Device-to-device copy, asynchronous on `stream`. */
+int fmmhip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int fmmhip_stream_create(void** stream);
 int fmmhip_stream_destroy(void* stream);
 int fmmhip_stream_sync(void* stream);
