@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=16384)
-    ap.add_argument("--alg", default="222")
+    ap.add_argument("--alg", default="222", help="222, 232, 322, 333, 444 or classical")
     ap.add_argument("--levels", type=int, default=2)
     ap.add_argument("--leaf", default="f64", choices=["f64", "f32", "bf16x3"])
     ap.add_argument("--pack", type=int, default=-1, help="0 fused, 1 pre-summed, -1 automatic")
@@ -107,11 +107,14 @@ def main():
     F.check(F.lib().fmmhip_init(), "init")
     n = args.size
     m = k = n
-    if args.alg == "222":
-        f = F.strassen()
+    if args.alg == "classical":
+        args.levels = 1
+        f = F.FmmT(1, 1, 1, 1)
+        alg = F.Alg(classical=True)
     else:
-        f = F.new_fmm(os.path.join(ROOT, "tests", "golden", "algorithms", f"{args.alg}.txt"))
-    alg = F.Alg([f] * args.levels)
+        f = F.strassen() if args.alg == "222" else \
+            F.new_fmm(os.path.join(ROOT, "tests", "golden", "algorithms", f"{args.alg}.txt"))
+        alg = F.Alg([f] * args.levels)
     alg.set_pack(args.pack)
     info = alg.info()
     leaf = {"f64": F.LEAF_NATIVE, "f32": F.LEAF_F32, "bf16x3": F.LEAF_BF16X3}[args.leaf]
