@@ -83,6 +83,117 @@ def cpu_baseline(seconds_target=15.0):
             "sample": f"oracle 1-level <2,2,2;7> Strassen FP64, 2048^3, C += A*B, min of {len(times)} runs"}
 
 
+def batch_shapes(count, seed):
+    """(dtype, m, n, k) of the batching workload: m, n, k log-uniform in
+    [256, 8192], multiples of 64; the first half FP64, the rest FP32."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def lu():
+        x = float(np.exp(rng.uniform(np.log(256.0), np.log(8192.0))))
+        return int(min(8192, max(256, round(x / 64.0) * 64)))
+    return [("f8" if i < count // 2 else "f4", lu(), lu(), lu()) for i in range(count)]
+
+
+def run_batch(args, F):
+    """The batching form: one captured graph over all problems with the
+    tuner's choices, beside the same graph with every problem classical and
+    beside one classical call per problem (no graph)."""
+    shapes = batch_shapes(args.batch_count, args.seed)
+    mats = []
+    for i, (dt, m, n, k) in enumerate(shapes):
+        A = F.DeviceMatrix(m, k, dtype=dt).fill_uniform(args.seed, 3 * i)
+        B = F.DeviceMatrix(k, n, dtype=dt).fill_uniform(args.seed, 3 * i + 1)
+        C = F.DeviceMatrix(m, n, dtype=dt).fill_uniform(args.seed, 3 * i + 2)
+        mats.append((A, B, C))
+    F.sync()
+    flops = sum(2.0 * m * n * k for (_, m, n, k) in shapes)
+    flops64 = sum(2.0 * m * n * k for (dt, m, n, k) in shapes if dt == "f8")
+
+    def make(cand):
+        b = F.Batch()
+        for (dt, m, n, k), (A, B, C) in zip(shapes, mats):
+            b.add(m, n, k, 1.0, A, B, 1.0, C, cand=cand)
+        t0 = time.perf_counter()
+        b.plan()
+        return b, time.perf_counter() - t0
+
+    def timed(run, steps, warmup):
+        for _ in range(warmup):
+            run()
+        F.sync()
+        evs = [(F.Event(), F.Event()) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record()
+            run()
+            e1.record()
+        F.sync()
+        wall = time.perf_counter() - t0
+        return wall * 1e3 / steps, [e0.elapsed_ms(e1) for e0, e1 in evs]
+
+    tuned, plan_s = make(-1)
+    info = tuned.info()
+    choices = {"f64": [0, 0, 0], "f32": [0, 0, 0]}
+    leaf64 = leaf32 = 0.0
+    for i, (dt, m, n, k) in enumerate(shapes):
+        c = tuned.choice(i)
+        choices["f64" if dt == "f8" else "f32"][c] += 1
+        d = 2 ** c
+        fl = 2.0 * (7 ** c) * -(-m // d) * -(-n // d) * -(-k // d)
+        if dt == "f8":
+            leaf64 += fl
+        else:
+            leaf32 += fl
+    ms, lat = timed(tuned.run, args.steps, args.warmup)
+
+    cls, _ = make(0)
+    cls_ms, _ = timed(cls.run, max(1, args.classical_steps), 1) if args.classical_steps > 0 else (None, None)
+    cls.close()
+
+    seq_ms = None
+    if args.classical_steps > 0:
+        def seq():
+            for (dt, m, n, k), (A, B, C) in zip(shapes, mats):
+                (F.dgemm if dt == "f8" else F.sgemm)(m, n, k, 1.0, A, B, 1.0, C)
+        seq_ms, _ = timed(seq, 1, 1)
+
+    # roofline of the persistent kernels: executed leaf flops over the step
+    # time against the flop-weighted MFMA peak of the two datatypes
+    t_peak = leaf64 / (FP64_PEAK_TFLOPS * 1e12) + leaf32 / (FP32_PEAK_TFLOPS * 1e12)
+    achieved = (leaf64 + leaf32) / (ms * 1e-3) / 1e12
+    peak = (leaf64 + leaf32) / t_peak / 1e12
+    eff = flops / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "effective GFLOPS (2mnk/t) of a 512-GEMM uneven FP64/FP32 batch, tuned FMM vs classical",
+        "value": round(eff, 1),
+        "unit": "GFLOPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": f"synthetic (SplitMix64 uniform[-1,1], seed {args.seed}, device-resident)",
+        "config": {
+            "workload": f"batch of {len(shapes)} GEMMs C += A*B, m,n,k log-uniform [256,8192] multiples of 64, "
+                        f"first half FP64, rest FP32, one captured HIP graph",
+            "problems": len(shapes), "total_flops": flops, "fp64_flops": flops64,
+            "choices_classical_1level_2level": choices,
+            "items_f64": info["items_f64"], "items_f32": info["items_f32"],
+            "workspace_bytes": info["workspace_bytes"], "tuner_est_ms": round(info["est_ms"], 3),
+            "plan_s": round(plan_s, 3),
+            "classical_batch_ms": round(cls_ms, 3) if cls_ms else None,
+            "classical_batch_gflops": round(flops / (cls_ms * 1e-3) / 1e9, 1) if cls_ms else None,
+            "tuned_over_classical": round(cls_ms / ms, 4) if cls_ms else None,
+            "sequential_classical_calls_ms": round(seq_ms, 3) if seq_ms else None,
+        },
+        "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": round(peak, 2), "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None, "kernel": "fmm_batch_kernel (f64, f32)",
+                     "leaf_flops_f64": leaf64, "leaf_flops_f32": leaf32},
+    }
+    print(json.dumps(out), flush=True)
+    tuned.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +208,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against the classical kernel")
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--form", default="single", choices=["single", "batch"],
+                    help="single: one m=n=k product; batch: 512 uneven FP64/FP32 GEMMs in one graph")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU sharding is a "
@@ -105,6 +220,8 @@ def main():
 
     import fmm_blis_amd as F
     F.check(F.lib().fmmhip_init(), "init")
+    if args.form == "batch":
+        return run_batch(args, F)
     n = args.size
     m = k = n
     if args.alg == "classical":

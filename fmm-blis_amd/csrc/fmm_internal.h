@@ -40,6 +40,7 @@ struct fmmhip_alg_s {
     int  max_nnz_u, max_nnz_v, max_nnz_w;
     int  shape;                 /* kernel launch shape, -1 = chosen per call */
     int  pack;                  /* 0 fused, 1 pre-summed workspace, -1 = chosen per call */
+    int  keep_all;              /* never evict cached descriptors (owners that capture graphs) */
     void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
 };
 
@@ -71,6 +72,22 @@ int fmmhip_scale_s(float* C, long m, long n, long rsC, long csC, float beta, voi
 int fmmhip_pointer_is_device(const void* p);
 /* The library's default stream (created on first use). */
 void* fmmhip_default_stream(void);
+/* One operand of one problem for the batched summation stage. */
+typedef struct fmmhip_presum_seg_s {
+    const void* X;              /* operand */
+    void* S;                    /* its workspace leaves */
+    long ld;                    /* operand leading dimension */
+    const int* ptr;             /* device CSR over products (R+1) */
+    const fmm_entry_t* ent;     /* device entries */
+    int R, swap, Lc, Ls, ldo;
+    int pad_;
+} fmmhip_presum_seg_t;
+
+/* Batched summation: segment s is processed by blocks [boff[s], boff[s+1]),
+ * boff has nseg+1 entries, nblocks = boff[nseg] (all device memory). */
+int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff, int nblocks,
+                        void* stream);
+
 /* Summation stage of the pre-summed pack mode (fmmhip_presum.hip). */
 int fmmhip_presum_d(const double* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,
                     int Lc, int Ls, int ldo, double* S, void* stream);

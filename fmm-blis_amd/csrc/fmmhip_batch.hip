@@ -1,0 +1,575 @@
+// fmmhip_batch.hip -- the batching form and its autotuner.
+//
+// Many independent column-major GEMMs (FP64 and FP32) run as one captured
+// HIP graph: per datatype, one batched summation launch (the pre-summed
+// operand sums of every FMM problem) and one persistent MFMA launch whose
+// workgroups take (problem, tile) items from a device queue until it is
+// empty.  The algorithm, level and pack mode of every problem are chosen by
+// the tuner below.  No counterpart in the reference (its driver runs one
+// product at a time, frame/base/bli_strassen_ab.c:123-295).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+#include <stdio.h>
+#include <string.h>
+
+#include "fmmhip_host.h"
+
+using fmmk::BatchItem;
+using fmmk::KArgs;
+
+namespace {
+
+// ---- tuner -------------------------------------------------------------------
+
+// Candidates: 0 classical, 1 one-level Strassen, 2 two-level Strassen (both
+// pre-summed).  Cost model of one candidate on one problem:
+//   t = F_leaf / (P * eff * nk/(nk + ovh)) * quant + bytes_sum / bw
+// F_leaf the executed leaf flops, P the MFMA peak of the datatype, eff the
+// kernel's sustained fraction of peak, nk the 16-wide k steps per product,
+// ovh the per-product prologue/epilogue cost in k steps, quant the
+// wave-quantisation loss of a lone launch (1 inside a batch), bytes_sum the
+// summation stage's traffic and bw its bandwidth.
+constexpr int kCands = 3;
+struct TunerParams {
+    double eff[2][kCands];      // [is_double][cand]
+    double ovh[2][kCands];
+    double bw;                  // summation bandwidth, bytes/s
+    double launch_ms;           // fixed cost per launch
+    int cus;
+};
+
+std::mutex g_tuner_mu;
+TunerParams g_tp = {
+    // defaults measured on MI355X (16384^3: classical 78%, pre-summed FMM kernel 72% of peak)
+    {{0.70, 0.66, 0.66}, {0.78, 0.72, 0.72}},
+    {{4.0, 4.0, 4.0}, {4.0, 4.0, 4.0}},
+    2.3e12, 0.01, 256};
+
+const double kPeak[2] = {157.3e12, 78.6e12};   // FP32, FP64 MFMA peaks (FLOP/s)
+
+struct CandShape {
+    int mt, kt, nt, R;
+};
+const CandShape kShape[kCands] = {{1, 1, 1, 1}, {2, 2, 2, 7}, {4, 4, 4, 49}};
+
+double est_ms(const TunerParams& tp, int is_double, int cand, long m, long n, long k, bool lone)
+{
+    const CandShape& c = kShape[cand];
+    const double lm = (double)((m + c.mt - 1) / c.mt), lk = (double)((k + c.kt - 1) / c.kt),
+                 ln = (double)((n + c.nt - 1) / c.nt);
+    const double fl = 2.0 * c.R * lm * lk * ln;
+    const double nk = std::ceil(lk / fmmk::BK);
+    double eff = tp.eff[is_double][cand] * nk / (nk + tp.ovh[is_double][cand]);
+    if (lone) {
+        const double tiles = std::ceil(lm / fmmk::BM) * std::ceil(ln / fmmk::BN);
+        const double slots = 2.0 * tp.cus;
+        const double waves = std::ceil(tiles / slots);
+        eff *= tiles / (waves * slots);
+    }
+    double t = fl / (kPeak[is_double] * eff);
+    if (cand > 0) {
+        const double esz = is_double ? 8.0 : 4.0;
+        const double bytes = esz * ((double)m * k + c.R * lm * lk + (double)k * n + c.R * lk * ln);
+        t += bytes / tp.bw;
+    }
+    return t * 1e3 + (lone ? tp.launch_ms * (cand > 0 ? 3 : 1) : 0.0);
+}
+
+// An FMM candidate must beat the best simpler one by this factor: the model's
+// error is of the order of a few percent, and a wrong FMM choice costs more
+// (workspace, a summation launch, long work items) than a wrong classical one.
+constexpr double kFmmMargin = 0.95;
+
+int choose(int is_double, long m, long n, long k, bool lone, double* best_ms)
+{
+    TunerParams tp;
+    {
+        std::lock_guard<std::mutex> lk(g_tuner_mu);
+        tp = g_tp;
+    }
+    int best = 0;
+    double bt = 1e300;
+    for (int c = 0; c < kCands; c++) {
+        // a level needs leaves of at least two tiles and k of 8 BK-steps
+        const CandShape& s = kShape[c];
+        if (c > 0 && (m < 2L * fmmk::BM * s.mt || n < 2L * fmmk::BN * s.nt || k < 8L * fmmk::BK * s.kt)) continue;
+        const double t = est_ms(tp, is_double, c, m, n, k, lone);
+        if (c == 0 || t < kFmmMargin * bt) { bt = t; best = c; }
+    }
+    if (best_ms) *best_ms = bt;
+    return best;
+}
+
+// The candidates' algorithms (shared by all batches).
+fmmhip_alg_t cand_alg(int cand)
+{
+    static std::once_flag once;
+    static fmmhip_alg_t algs[kCands] = {nullptr, nullptr, nullptr};
+    std::call_once(once, [] {
+        algs[0] = classical_alg();
+        fmmhip_alg_create(&algs[1], &STRASSEN_FMM, 1);
+        fmm_t two[2] = {STRASSEN_FMM, STRASSEN_FMM};
+        fmmhip_alg_create(&algs[2], two, 2);
+    });
+    return algs[cand];
+}
+
+}  // namespace
+
+extern "C" int fmmhip_tuner_choose(int is_double, long m, long n, long k, int lone, int* cand, double* est)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || !cand) return FMMHIP_ERR_INVALID;
+    *cand = choose(is_double ? 1 : 0, m, n, k, lone != 0, est);
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, long k, int lone, double* est)
+{
+    if (cand < 0 || cand >= kCands || !est) return FMMHIP_ERR_INVALID;
+    TunerParams tp;
+    {
+        std::lock_guard<std::mutex> lk(g_tuner_mu);
+        tp = g_tp;
+    }
+    *est = est_ms(tp, is_double ? 1 : 0, cand, m, n, k, lone != 0);
+    return FMMHIP_SUCCESS;
+}
+
+// Times each candidate on a few square sizes with the library's launch
+// timing and refits eff (largest size), ovh (smallest size) and bw.
+extern "C" int fmmhip_tuner_calibrate(void* stream)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
+    int cus = 256;
+    fmmhip_device_cu_count(&cus);
+    const long big = 4096, small = 1024;
+    TunerParams tp;
+    {
+        std::lock_guard<std::mutex> lk(g_tuner_mu);
+        tp = g_tp;
+    }
+    tp.cus = cus;
+    double bw_sum = 0.0, bw_n = 0.0;
+    for (int dbl = 0; dbl < 2; dbl++) {
+        const size_t esz = dbl ? 8 : 4;
+        void *A = nullptr, *B = nullptr, *C = nullptr;
+        if (fmmhip_malloc(&A, esz * big * big) || fmmhip_malloc(&B, esz * big * big) ||
+            fmmhip_malloc(&C, esz * big * big)) {
+            fmmhip_free(A); fmmhip_free(B); fmmhip_free(C);
+            return FMMHIP_ERR_NOMEM;
+        }
+        if (dbl) {
+            fmmhip_fill_uniform((double*)A, big, big, big, 7, 0, st);
+            fmmhip_fill_uniform((double*)B, big, big, big, 7, 1, st);
+            fmmhip_fill_uniform((double*)C, big, big, big, 7, 2, st);
+        } else {
+            fmmhip_fill_uniform_f32((float*)A, big, big, big, 7, 0, st);
+            fmmhip_fill_uniform_f32((float*)B, big, big, big, 7, 1, st);
+            fmmhip_fill_uniform_f32((float*)C, big, big, big, 7, 2, st);
+        }
+        for (int cand = 0; cand < kCands; cand++) {
+            fmmhip_alg_t alg = cand_alg(cand);
+            const int saved_pack = alg->pack;
+            alg->pack = cand > 0 ? 1 : 0;
+            double meas[2] = {0, 0}, pres[2] = {0, 0};
+            const long sizes[2] = {big, small};
+            for (int si = 0; si < 2; si++) {
+                const long nn = sizes[si];
+                for (int rep = 0; rep < 3; rep++) {
+                    if (rep == 1) fmmhip_timing_enable(1);
+                    if (dbl)
+                        rc = fmmhip_dgemm_fmm(alg, FMMHIP_LEAF_NATIVE, nn, nn, nn, 1.0, (double*)A, 1, big,
+                                              (double*)B, 1, big, 1.0, (double*)C, 1, big, st);
+                    else
+                        rc = fmmhip_sgemm_fmm(alg, nn, nn, nn, 1.0f, (float*)A, 1, big, (float*)B, 1, big, 1.0f,
+                                              (float*)C, 1, big, st);
+                    if (rc) break;
+                }
+                fmmhip_timing_enable(0);
+                long nl = 0;
+                double kms = 0, pms = 0, mx = 0;
+                fmmhip_timing_read(&nl, &kms, &pms, &mx);
+                if (rc || nl <= 0) break;
+                meas[si] = kms / nl;
+                pres[si] = pms / nl;
+            }
+            alg->pack = saved_pack;
+            if (rc) break;
+            const CandShape& c = kShape[cand];
+            // eff from the big size (kernel only); ovh from the small one
+            const double lb = (double)big / c.mt, ls = (double)small / c.mt;
+            const double flb = 2.0 * c.R * lb * lb * lb, fls = 2.0 * c.R * ls * ls * ls;
+            const double nkb = std::ceil(lb / fmmk::BK), nks = std::ceil(ls / fmmk::BK);
+            const double tiles_s = std::ceil(ls / fmmk::BM) * std::ceil(ls / fmmk::BN);
+            const double q_s = tiles_s / (std::ceil(tiles_s / (2.0 * cus)) * 2.0 * cus);
+            const double tiles_b = std::ceil(lb / fmmk::BM) * std::ceil(lb / fmmk::BN);
+            const double q_b = tiles_b / (std::ceil(tiles_b / (2.0 * cus)) * 2.0 * cus);
+            const double eb = flb / (kPeak[dbl] * meas[0] * 1e-3) / q_b;   // = eff*nkb/(nkb+ovh)
+            const double es = fls / (kPeak[dbl] * meas[1] * 1e-3) / q_s;
+            // solve eff*nk/(nk+ovh) = e at both sizes
+            double ovh = 4.0, eff = eb;
+            if (es > 0 && eb > 0 && es < eb) {
+                ovh = (eb - es) / (es / nks - eb / nkb);
+                if (!(ovh > 0) || ovh > 64) ovh = 4.0;
+                eff = eb * (nkb + ovh) / nkb;
+            }
+            if (eff > 0.98) eff = 0.98;
+            tp.eff[dbl][cand] = eff;
+            tp.ovh[dbl][cand] = ovh;
+            if (cand > 0 && pres[0] > 0) {
+                const double bytes = (double)esz * (2.0 * big * big + 2.0 * c.R * lb * lb);
+                bw_sum += bytes / (pres[0] * 1e-3);
+                bw_n += 1;
+            }
+        }
+        fmmhip_free(A); fmmhip_free(B); fmmhip_free(C);
+        if (rc) return rc;
+    }
+    if (bw_n > 0) tp.bw = bw_sum / bw_n;
+    std::lock_guard<std::mutex> lk(g_tuner_mu);
+    g_tp = tp;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_tuner_save(const char* path)
+{
+    if (!path) return FMMHIP_ERR_INVALID;
+    TunerParams tp;
+    {
+        std::lock_guard<std::mutex> lk(g_tuner_mu);
+        tp = g_tp;
+    }
+    FILE* f = fopen(path, "w");
+    if (!f) return FMMHIP_ERR_IO;
+    fprintf(f, "# fmm_blis_amd tuner table: candidate dtype eff ovh (candidates 0 classical, 1 222x1, 2 222x2)\n");
+    for (int d = 0; d < 2; d++)
+        for (int c = 0; c < kCands; c++) fprintf(f, "cand %d %s %.6f %.6f\n", c, d ? "f64" : "f32", tp.eff[d][c], tp.ovh[d][c]);
+    fprintf(f, "bw %.6e\nlaunch_ms %.6f\ncus %d\n", tp.bw, tp.launch_ms, tp.cus);
+    fclose(f);
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_tuner_load(const char* path)
+{
+    if (!path) return FMMHIP_ERR_INVALID;
+    FILE* f = fopen(path, "r");
+    if (!f) return FMMHIP_ERR_IO;
+    TunerParams tp;
+    {
+        std::lock_guard<std::mutex> lk(g_tuner_mu);
+        tp = g_tp;
+    }
+    char line[256];
+    int ok = 0;
+    while (fgets(line, sizeof line, f)) {
+        int c;
+        char dt[8];
+        double e, o;
+        if (sscanf(line, "cand %d %7s %lf %lf", &c, dt, &e, &o) == 4 && c >= 0 && c < kCands && e > 0 && e <= 1 &&
+            o >= 0) {
+            const int d = strcmp(dt, "f64") == 0 ? 1 : 0;
+            tp.eff[d][c] = e;
+            tp.ovh[d][c] = o;
+            ok++;
+        } else if (sscanf(line, "bw %lf", &e) == 1 && e > 0) {
+            tp.bw = e;
+        } else if (sscanf(line, "launch_ms %lf", &e) == 1 && e >= 0) {
+            tp.launch_ms = e;
+        } else if (sscanf(line, "cus %d", &c) == 1 && c > 0) {
+            tp.cus = c;
+        }
+    }
+    fclose(f);
+    if (ok == 0) return FMMHIP_ERR_IO;
+    std::lock_guard<std::mutex> lk(g_tuner_mu);
+    g_tp = tp;
+    return FMMHIP_SUCCESS;
+}
+
+// ---- batch ------------------------------------------------------------------
+
+struct fmmhip_batch_s {
+    struct Prob {
+        int dbl;
+        long m, n, k;
+        double alpha, beta;
+        const void* A;
+        long lda;
+        const void* B;
+        long ldb;
+        void* C;
+        long ldc;
+        int force;              // -1 tuner, else candidate
+        int cand;               // chosen
+    };
+    std::vector<Prob> probs;
+    std::vector<Prepared> prep;
+    // the batch's own algorithms: their descriptor caches never evict, so
+    // every pointer captured in the graph stays valid for the batch's life
+    fmmhip_alg_t algs[kCands] = {nullptr, nullptr, nullptr};
+    hipStream_t stream = nullptr;
+    void* ws = nullptr;
+    void* dev = nullptr;        // KArgs, items, segments, block offsets, heads
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool planned = false;
+    double est_ms = 0.0;
+    long items[2] = {0, 0};
+
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        if (ws || dev) (void)hipDeviceSynchronize();
+        if (ws) (void)hipFree(ws);
+        if (dev) (void)hipFree(dev);
+        ws = dev = nullptr;
+        for (int c = 0; c < kCands; c++) {
+            fmmhip_alg_destroy(algs[c]);
+            algs[c] = nullptr;
+        }
+        planned = false;
+    }
+
+    int make_algs()
+    {
+        int rc = fmmhip_alg_create_classical(&algs[0]);
+        if (!rc) rc = fmmhip_alg_create(&algs[1], &STRASSEN_FMM, 1);
+        fmm_t two[2] = {STRASSEN_FMM, STRASSEN_FMM};
+        if (!rc) rc = fmmhip_alg_create(&algs[2], two, 2);
+        for (int c = 0; c < kCands && !rc; c++) algs[c]->keep_all = 1;
+        return rc;
+    }
+};
+
+extern "C" int fmmhip_batch_create(fmmhip_batch_t* out)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    if (!out) return FMMHIP_ERR_INVALID;
+    fmmhip_batch_t b = new fmmhip_batch_s();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return FMMHIP_ERR_HIP;
+    }
+    *out = b;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" void fmmhip_batch_destroy(fmmhip_batch_t b)
+{
+    if (!b) return;
+    b->release();
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+static int batch_add(fmmhip_batch_t b, int dbl, long m, long n, long k, double alpha, const void* A, long lda,
+                     const void* B, long ldb, double beta, void* C, long ldc, int force)
+{
+    if (!b || m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) ||
+        ldc < (m > 1 ? m : 1) || force < -1 || force >= kCands)
+        return FMMHIP_ERR_INVALID;
+    b->release();
+    fmmhip_batch_s::Prob p{dbl, m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, force, -1};
+    b->probs.push_back(p);
+    return (int)b->probs.size() - 1 >= 0 ? FMMHIP_SUCCESS : FMMHIP_ERR_INVALID;
+}
+
+extern "C" int fmmhip_batch_add_d(fmmhip_batch_t b, long m, long n, long k, double alpha, const double* A, long lda,
+                                  const double* B, long ldb, double beta, double* C, long ldc, int cand)
+{
+    return batch_add(b, 1, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cand);
+}
+
+extern "C" int fmmhip_batch_add_s(fmmhip_batch_t b, long m, long n, long k, float alpha, const float* A, long lda,
+                                  const float* B, long ldb, float beta, float* C, long ldc, int cand)
+{
+    return batch_add(b, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cand);
+}
+
+extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
+{
+    if (!b) return FMMHIP_ERR_INVALID;
+    b->release();
+    int rc = b->make_algs();
+    if (rc) return rc;
+    const size_t np = b->probs.size();
+    b->prep.assign(np, Prepared());
+    b->est_ms = 0.0;
+    // 1. choices and preparation (descriptors uploaded on the batch stream)
+    size_t ws_total = 0;
+    std::vector<size_t> ws_off(np, 0);
+    for (size_t i = 0; i < np && !rc; i++) {
+        fmmhip_batch_s::Prob& p = b->probs[i];
+        double est = 0.0;
+        if (p.force >= 0) {
+            p.cand = p.force;
+            if (p.m > 0 && p.n > 0 && p.k > 0) fmmhip_tuner_estimate(p.dbl, p.cand, p.m, p.n, p.k, 0, &est);
+        } else if (p.m > 0 && p.n > 0 && p.k > 0) {
+            p.cand = choose(p.dbl, p.m, p.n, p.k, false, &est);
+        } else {
+            p.cand = 0;
+        }
+        b->est_ms += est;
+        rc = prepare_call(b->algs[p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1, p.lda,
+                          p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? 1 : 0, b->stream, &b->prep[i]);
+        // the batched kernel is instantiated for column-major operands only
+        if (!rc && !b->prep[i].nothing && !b->prep[i].scale_only &&
+            !(b->prep[i].a_mn && !b->prep[i].b_mn && b->prep[i].c_col))
+            rc = FMMHIP_ERR_UNSUPPORTED;
+        if (!rc && b->prep[i].presum) {
+            ws_off[i] = ws_total;
+            ws_total += (b->prep[i].ws_bytes + 255) & ~(size_t)255;
+        }
+    }
+    if (rc) return rc;
+    if (ws_total) {
+        if (hipMalloc(&b->ws, ws_total) != hipSuccess) {
+            (void)hipGetLastError();
+            b->ws = nullptr;
+            return FMMHIP_ERR_NOMEM;
+        }
+        for (size_t i = 0; i < np; i++)
+            if (b->prep[i].presum) bind_workspace(&b->prep[i], (char*)b->ws + ws_off[i]);
+    }
+    // 2. per datatype: kernel arguments, work items (costliest first),
+    //    summation segments and their block offsets
+    std::vector<KArgs> ka[2];
+    std::vector<BatchItem> items[2];
+    std::vector<double> cost[2];
+    std::vector<fmmhip_presum_seg_t> segs[2];
+    std::vector<int> boff[2];
+    bool vec[2] = {true, true};
+    std::vector<size_t> scale_only;
+    for (int d = 0; d < 2; d++) boff[d].push_back(0);
+    for (size_t i = 0; i < np; i++) {
+        const Prepared& P = b->prep[i];
+        const int d = b->probs[i].dbl;
+        if (P.nothing) continue;
+        if (P.scale_only) { scale_only.push_back(i); continue; }
+        const int pi = (int)ka[d].size();
+        ka[d].push_back(P.ka);
+        vec[d] = vec[d] && P.ka.va && P.ka.vb;
+        const double c = (double)P.ka.R * (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
+        for (long t = 0; t < P.nwg; t++) {
+            items[d].push_back(BatchItem{pi, (int)t});
+            cost[d].push_back(c);
+        }
+        if (P.presum) {
+            for (const fmmhip_presum_seg_t* sg : {&P.segA, &P.segB}) {
+                const long pairs = (long)((sg->Lc + 1) / 2) * sg->Ls;
+                const int nbl = (int)std::max(1L, (pairs + 256L * 16 - 1) / (256L * 16));
+                segs[d].push_back(*sg);
+                boff[d].push_back(boff[d].back() + nbl);
+            }
+        }
+    }
+    for (int d = 0; d < 2; d++) {
+        std::vector<size_t> order(items[d].size());
+        for (size_t j = 0; j < order.size(); j++) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[d][x] > cost[d][y]; });
+        std::vector<BatchItem> sorted(order.size());
+        for (size_t j = 0; j < order.size(); j++) sorted[j] = items[d][order[j]];
+        items[d].swap(sorted);
+        b->items[d] = (long)items[d].size();
+    }
+    // 3. one device block: [ka0 | ka1 | items0 | items1 | segs0 | segs1 | boff0 | boff1 | head0 head1]
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) & ~(size_t)255; return o; };
+    size_t o_ka[2], o_it[2], o_sg[2], o_bo[2];
+    for (int d = 0; d < 2; d++) o_ka[d] = take(sizeof(KArgs) * (ka[d].size() + 1));
+    for (int d = 0; d < 2; d++) o_it[d] = take(sizeof(BatchItem) * (items[d].size() + 1));
+    for (int d = 0; d < 2; d++) o_sg[d] = take(sizeof(fmmhip_presum_seg_t) * (segs[d].size() + 1));
+    for (int d = 0; d < 2; d++) o_bo[d] = take(sizeof(int) * (boff[d].size() + 1));
+    const size_t o_head = take(2 * sizeof(int));
+    if (hipMalloc(&b->dev, off) != hipSuccess) { (void)hipGetLastError(); b->dev = nullptr; return FMMHIP_ERR_NOMEM; }
+    char* dv = (char*)b->dev;
+    for (int d = 0; d < 2; d++) {
+        if (!ka[d].empty() &&
+            hipMemcpy(dv + o_ka[d], ka[d].data(), sizeof(KArgs) * ka[d].size(), hipMemcpyHostToDevice) != hipSuccess)
+            return FMMHIP_ERR_HIP;
+        if (!items[d].empty() && hipMemcpy(dv + o_it[d], items[d].data(), sizeof(BatchItem) * items[d].size(),
+                                           hipMemcpyHostToDevice) != hipSuccess)
+            return FMMHIP_ERR_HIP;
+        if (!segs[d].empty() && hipMemcpy(dv + o_sg[d], segs[d].data(), sizeof(fmmhip_presum_seg_t) * segs[d].size(),
+                                          hipMemcpyHostToDevice) != hipSuccess)
+            return FMMHIP_ERR_HIP;
+        if (hipMemcpy(dv + o_bo[d], boff[d].data(), sizeof(int) * boff[d].size(), hipMemcpyHostToDevice) != hipSuccess)
+            return FMMHIP_ERR_HIP;
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return FMMHIP_ERR_HIP;   // descriptor uploads
+    // 4. capture: summation, then the persistent kernel, per datatype
+    int cus = 256;
+    fmmhip_device_cu_count(&cus);
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FMMHIP_ERR_HIP;
+    for (size_t idx : scale_only) {
+        const Prepared& P = b->prep[idx];
+        if (P.kind == Kind::S32) rc = fmmhip_scale_s((float*)P.C, P.m, P.n, P.rsC, P.csC, (float)P.beta, b->stream);
+        else rc = fmmhip_scale_d((double*)P.C, P.m, P.n, P.rsC, P.csC, P.beta, b->stream);
+        if (rc) break;
+    }
+    for (int d = 1; d >= 0 && !rc; d--) {
+        if (!segs[d].empty())
+            rc = fmmhip_presum_batch(d, (const fmmhip_presum_seg_t*)(dv + o_sg[d]), (int)segs[d].size(),
+                                     (const int*)(dv + o_bo[d]), boff[d].back(), b->stream);
+        if (!rc && !items[d].empty()) {
+            const int nwg = (int)std::min<long>((long)items[d].size(), 2L * cus);
+            int* head = (int*)(dv + o_head) + d;
+            if (d)
+                rc = fmmk_launch_batch_d64((const KArgs*)(dv + o_ka[d]), (const BatchItem*)(dv + o_it[d]),
+                                           (int)items[d].size(), head, vec[d], nwg, b->stream);
+            else
+                rc = fmmk_launch_batch_s32((const KArgs*)(dv + o_ka[d]), (const BatchItem*)(dv + o_it[d]),
+                                           (int)items[d].size(), head, vec[d], nwg, b->stream);
+        }
+    }
+    hipGraph_t g = nullptr;
+    const bool cap_ok = hipStreamEndCapture(b->stream, &g) == hipSuccess;
+    if (rc || !cap_ok) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : FMMHIP_ERR_HIP;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return FMMHIP_ERR_HIP;
+    b->planned = true;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_batch_run(fmmhip_batch_t b, void* stream)
+{
+    if (!b || !b->planned) return FMMHIP_ERR_INVALID;
+    hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
+    return hipGraphLaunch(b->exec, st) == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+}
+
+extern "C" int fmmhip_batch_info(fmmhip_batch_t b, long* nprob, long* items_f64, long* items_f32, double* est_ms,
+                                 size_t* ws_bytes)
+{
+    if (!b) return FMMHIP_ERR_INVALID;
+    if (nprob) *nprob = (long)b->probs.size();
+    if (items_f64) *items_f64 = b->items[1];
+    if (items_f32) *items_f32 = b->items[0];
+    if (est_ms) *est_ms = b->est_ms;
+    if (ws_bytes) {
+        size_t t = 0;
+        for (const Prepared& P : b->prep)
+            if (P.presum) t += (P.ws_bytes + 255) & ~(size_t)255;
+        *ws_bytes = t;
+    }
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_batch_choice(fmmhip_batch_t b, long i, int* cand)
+{
+    if (!b || i < 0 || (size_t)i >= b->probs.size() || !cand) return FMMHIP_ERR_INVALID;
+    *cand = b->probs[i].cand;
+    return FMMHIP_SUCCESS;
+}

@@ -155,7 +155,7 @@ int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool a_mn,
     for (DevDesc* c : cache->items)
         if (same_geo(c->geo, g)) { d = c; break; }
     if (!d) {
-        if (cache->items.size() >= kCacheCap) {
+        if (!alg->keep_all && cache->items.size() >= kCacheCap) {
             // evict the least recently used descriptor; kernels may still read it
             size_t victim = 0;
             for (size_t i = 1; i < cache->items.size(); i++)
@@ -212,6 +212,10 @@ void* workspace_get(hipStream_t st, size_t bytes)
     return p;
 }
 
+}  // namespace
+#include "fmmhip_host.h"
+namespace {
+
 // Pairs along the contiguous dimension can be loaded as one 16-byte vector
 // when the base, the leading dimension and every part offset keep them
 // 16-byte aligned (pairs start at even indices of the tile).
@@ -225,8 +229,10 @@ bool pairs_aligned(const void* base, size_t esz, long ld, const fmm_entry_t* ent
     return true;
 }
 
-enum class Kind { D64, D_F32LEAF, S32 };
-
+// Everything one call needs, resolved on the host: the kernel arguments,
+// the launch shape and, in the pre-summed mode, the two summation segments.
+// Workspace pointers (segment S and the kernel's A/B) are offsets from the
+// workspace base until bind_workspace().
 // Optional per-launch timing (fmmhip_timing_*): HIP events recorded on the
 // launch stream around the summation stage and around the MFMA kernel.
 struct TimedCall {
@@ -246,25 +252,29 @@ void tm_record(hipEvent_t* ev, hipStream_t st)
 // Largest workspace the automatic choice takes for the pre-summed mode.
 constexpr double kWsFractionOfFree = 0.5;
 
-int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha,
-                const void* A, long rsA, long csA, const void* B, long rsB, long csB, double beta,
-                void* C, long rsC, long csC, void* stream)
+}  // namespace
+
+int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
+                 long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
+                 hipStream_t st, Prepared* P)
 {
-    int rc = fmmhip_init();
-    if (rc) return rc;
+    memset(P, 0, sizeof(*P));
+    P->kind = kind;
+    P->stream = st;
     if (!alg || m < 0 || n < 0 || k < 0) return FMMHIP_ERR_INVALID;
-    hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
-    if (m == 0 || n == 0) return FMMHIP_SUCCESS;
+    if (m == 0 || n == 0) { P->nothing = true; return FMMHIP_SUCCESS; }
     if (!C) return FMMHIP_ERR_INVALID;
     if (k == 0 || alpha == 0.0) {
-        if (kind == Kind::S32) return fmmhip_scale_s((float*)C, m, n, rsC, csC, (float)beta, st);
-        return fmmhip_scale_d((double*)C, m, n, rsC, csC, beta, st);
+        P->scale_only = true;
+        P->C = C; P->m = m; P->n = n; P->rsC = rsC; P->csC = csC; P->beta = beta;
+        return FMMHIP_SUCCESS;
     }
     if (!A || !B) return FMMHIP_ERR_INVALID;
-    // operand layouts: one of the strides must be 1
+    // operand layouts: one of the strides must be 1; when both are (a
+    // dimension of 1), the column-major reading is taken
     bool a_mn, b_mn;
     if (rsA == 1) a_mn = true; else if (csA == 1) a_mn = false; else return FMMHIP_ERR_UNSUPPORTED;
-    if (csB == 1) b_mn = true; else if (rsB == 1) b_mn = false; else return FMMHIP_ERR_UNSUPPORTED;
+    if (rsB == 1) b_mn = false; else if (csB == 1) b_mn = true; else return FMMHIP_ERR_UNSUPPORTED;
     if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
     // per-thread tile offsets are 32-bit: (tile extent) x (stride) must fit
     const long kStrideMax = 1L << 23;
@@ -285,26 +295,20 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
     // pack mode: pre-summed when asked, or (automatic) for multi-part
     // algorithms when the workspace fits in half of the free device memory
     const WsLayout wa = ws_layout_a(g, a_mn), wb = ws_layout_b(g, b_mn);
-    const size_t ws_bytes = esz * (wa.elems(alg->R) + wb.elems(alg->R) + 64);
-    bool presum = alg->pack == 1;
-    if (alg->pack < 0 && multi) {
+    const size_t bytesA = esz * wa.elems(alg->R);
+    const size_t offB = (bytesA + 255) & ~(size_t)255;
+    const size_t ws_bytes = offB + esz * wb.elems(alg->R) + 256;
+    bool presum = pack == 1;
+    if (pack < 0 && multi) {
         size_t fr = 0, tot = 0;
         presum = hipMemGetInfo(&fr, &tot) == hipSuccess && ws_bytes < kWsFractionOfFree * (double)fr;
     }
-    void* ws = nullptr;
-    if (presum) {
-        ws = workspace_get(st, ws_bytes);
-        if (!ws) {
-            if (alg->pack == 1) return FMMHIP_ERR_NOMEM;
-            presum = false;          // automatic choice falls back to the fused mode
-        }
-    }
 
     DevDesc* d = nullptr;
-    rc = get_desc(alg, g, presum, a_mn, b_mn, st, &d);
+    int rc = get_desc(alg, g, presum, a_mn, b_mn, st, &d);
     if (rc) return rc;
 
-    KArgs ka;
+    KArgs& ka = P->ka;
     ka.C = C;
     ka.rsC = rsC; ka.csC = csC;
     ka.lm = (int)g.lm; ka.lk = (int)g.lk; ka.ln = (int)g.ln;
@@ -315,33 +319,20 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
     ka.alpha = alpha;
     ka.beta = beta;
     int shape = alg->shape;
-    bool timed;
-    {
-        std::lock_guard<std::mutex> lk(g_tm_mu);
-        timed = g_tm_on;
-    }
-    TimedCall tc{nullptr, nullptr, nullptr, presum};
-    if (timed) tm_record(&tc.e0, st);
     if (presum) {
-        // summation stage: S_r and T_r for every product into the workspace
-        char* S = (char*)ws;
-        char* T = S + esz * wa.elems(alg->R);
-        T = (char*)(((uintptr_t)T + 255) & ~(uintptr_t)255);
+        P->presum = true;
+        P->ws_bytes = ws_bytes;
         const int R = alg->R;
-        if (kind == Kind::S32) {
-            rc = fmmhip_presum_s((const float*)A, a_mn ? csA : rsA, d->fused.dptr(), d->fused.dent(), R,
-                                 a_mn ? 0 : 1, wa.Lc, wa.Ls, wa.ldo, (float*)S, st);
-            if (!rc) rc = fmmhip_presum_s((const float*)B, b_mn ? rsB : csB, d->fused.dptr() + (R + 1),
-                                          d->fused.dent(), R, b_mn ? 1 : 0, wb.Lc, wb.Ls, wb.ldo, (float*)T, st);
-        } else {
-            rc = fmmhip_presum_d((const double*)A, a_mn ? csA : rsA, d->fused.dptr(), d->fused.dent(), R,
-                                 a_mn ? 0 : 1, wa.Lc, wa.Ls, wa.ldo, (double*)S, st);
-            if (!rc) rc = fmmhip_presum_d((const double*)B, b_mn ? rsB : csB, d->fused.dptr() + (R + 1),
-                                          d->fused.dent(), R, b_mn ? 1 : 0, wb.Lc, wb.Ls, wb.ldo, (double*)T, st);
-        }
-        if (rc) return rc;
-        ka.A = S;
-        ka.B = T;
+        fmmhip_presum_seg_t& sa = P->segA;
+        sa.X = A; sa.S = (void*)(uintptr_t)0; sa.ld = a_mn ? csA : rsA;
+        sa.ptr = d->fused.dptr(); sa.ent = d->fused.dent();
+        sa.R = R; sa.swap = a_mn ? 0 : 1; sa.Lc = wa.Lc; sa.Ls = wa.Ls; sa.ldo = wa.ldo;
+        fmmhip_presum_seg_t& sb = P->segB;
+        sb.X = B; sb.S = (void*)(uintptr_t)offB; sb.ld = b_mn ? rsB : csB;
+        sb.ptr = d->fused.dptr() + (R + 1); sb.ent = d->fused.dent();
+        sb.R = R; sb.swap = b_mn ? 1 : 0; sb.Lc = wb.Lc; sb.Ls = wb.Ls; sb.ldo = wb.ldo;
+        ka.A = (const void*)(uintptr_t)0;
+        ka.B = (const void*)(uintptr_t)offB;
         ka.lda = wa.ldo;
         ka.ldb = wb.ldo;
         ka.min_m = (int)g.lm; ka.min_k = (int)g.lk; ka.min_n = (int)g.ln;   // full leaves
@@ -361,16 +352,66 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
         ka.vb = pairs_aligned(B, esz, ka.ldb, d->fused.ent_h, alg->nnz_u, alg->nnz_u + alg->nnz_v);
         if (shape < 0) shape = multi ? 1 : 0;
     }
-    long nwg = (long)ka.tiles_m * ka.tiles_n;
-    if (nwg <= 0) return FMMHIP_SUCCESS;
-    if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+    P->shape = shape;
+    P->a_mn = a_mn;
+    P->b_mn = b_mn;
+    P->c_col = rsC == 1;
+    P->nwg = (long)ka.tiles_m * ka.tiles_n;
+    if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+    if (P->nwg <= 0) P->nothing = true;
+    return FMMHIP_SUCCESS;
+}
 
-    const bool c_col = rsC == 1;
+void bind_workspace(Prepared* P, void* ws)
+{
+    if (!P->presum) return;
+    char* base = (char*)ws;
+    P->segA.S = base + (uintptr_t)P->segA.S;
+    P->segB.S = base + (uintptr_t)P->segB.S;
+    P->ka.A = P->segA.S;
+    P->ka.B = P->segB.S;
+}
+
+// Launches the summation stage (if any) and the MFMA kernel of a prepared,
+// workspace-bound call on its stream.
+int launch_prepared(const Prepared* P)
+{
+    hipStream_t st = P->stream;
+    if (P->nothing) return FMMHIP_SUCCESS;
+    if (P->scale_only) {
+        if (P->kind == Kind::S32) return fmmhip_scale_s((float*)P->C, P->m, P->n, P->rsC, P->csC, (float)P->beta, st);
+        return fmmhip_scale_d((double*)P->C, P->m, P->n, P->rsC, P->csC, P->beta, st);
+    }
+    bool timed;
+    {
+        std::lock_guard<std::mutex> lk(g_tm_mu);
+        timed = g_tm_on;
+    }
+    TimedCall tc{nullptr, nullptr, nullptr, P->presum};
+    if (timed) tm_record(&tc.e0, st);
+    int rc = FMMHIP_SUCCESS;
+    if (P->presum) {
+        const fmmhip_presum_seg_t& a = P->segA;
+        const fmmhip_presum_seg_t& b = P->segB;
+        if (P->kind == Kind::S32) {
+            rc = fmmhip_presum_s((const float*)a.X, a.ld, a.ptr, a.ent, a.R, a.swap, a.Lc, a.Ls, a.ldo, (float*)a.S, st);
+            if (!rc) rc = fmmhip_presum_s((const float*)b.X, b.ld, b.ptr, b.ent, b.R, b.swap, b.Lc, b.Ls, b.ldo,
+                                          (float*)b.S, st);
+        } else {
+            rc = fmmhip_presum_d((const double*)a.X, a.ld, a.ptr, a.ent, a.R, a.swap, a.Lc, a.Ls, a.ldo, (double*)a.S,
+                                 st);
+            if (!rc) rc = fmmhip_presum_d((const double*)b.X, b.ld, b.ptr, b.ent, b.R, b.swap, b.Lc, b.Ls, b.ldo,
+                                          (double*)b.S, st);
+        }
+        if (rc) return rc;
+    }
     if (timed) tm_record(&tc.e1, st);
-    switch (kind) {
-    case Kind::D64: rc = fmmk_launch_d64(ka, shape, a_mn, b_mn, c_col, (int)nwg, st); break;
-    case Kind::D_F32LEAF: rc = fmmk_launch_d_f32leaf(ka, shape, a_mn, b_mn, c_col, (int)nwg, st); break;
-    case Kind::S32: rc = fmmk_launch_s32(ka, shape, a_mn, b_mn, c_col, (int)nwg, st); break;
+    switch (P->kind) {
+    case Kind::D64: rc = fmmk_launch_d64(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
+    case Kind::D_F32LEAF:
+        rc = fmmk_launch_d_f32leaf(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st);
+        break;
+    case Kind::S32: rc = fmmk_launch_s32(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
     }
     if (timed) {
         tm_record(&tc.e2, st);
@@ -378,6 +419,32 @@ int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alph
         g_tm.push_back(tc);
     }
     return rc;
+}
+
+static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha,
+                const void* A, long rsA, long csA, const void* B, long rsB, long csB, double beta,
+                void* C, long rsC, long csC, void* stream)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
+    Prepared P;
+    if (!alg) return FMMHIP_ERR_INVALID;
+    const int pack = alg->pack;
+    rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, pack, st, &P);
+    if (rc) return rc;
+    if (P.presum) {
+        void* ws = workspace_get(st, P.ws_bytes);
+        if (!ws) {
+            if (pack == 1) return FMMHIP_ERR_NOMEM;
+            // automatic choice falls back to the fused mode
+            rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, 0, st, &P);
+            if (rc) return rc;
+        } else {
+            bind_workspace(&P, ws);
+        }
+    }
+    return launch_prepared(&P);
 }
 
 fmmhip_alg_t classical_alg()
@@ -388,7 +455,6 @@ fmmhip_alg_t classical_alg()
     return alg;
 }
 
-}  // namespace
 
 extern "C" int fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape)
 {

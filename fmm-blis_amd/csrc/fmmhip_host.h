@@ -1,0 +1,40 @@
+// fmmhip_host.h -- host-side call preparation shared by the one-problem
+// entry points (fmmhip_gemm.hip) and the batched form (fmmhip_batch.hip).
+#ifndef FMMHIP_HOST_H
+#define FMMHIP_HOST_H
+
+#include <hip/hip_runtime.h>
+
+#include "fmmhip_kernel.h"
+
+enum class Kind { D64, D_F32LEAF, S32 };
+
+struct Prepared {
+    Kind kind;
+    hipStream_t stream;
+    bool nothing;                // empty problem
+    bool scale_only;             // k == 0 or alpha == 0: C := beta*C
+    void* C;
+    long m, n, rsC, csC;
+    double beta;
+    fmmk::KArgs ka;              // kernel arguments (A/B are workspace offsets until bound)
+    int shape;
+    bool a_mn, b_mn, c_col;
+    long nwg;
+    bool presum;                 // pre-summed pack mode
+    size_t ws_bytes;             // its workspace
+    fmmhip_presum_seg_t segA, segB;
+};
+
+// Resolves one call (pack: 0 fused, 1 pre-summed, -1 automatic).
+int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
+                 long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
+                 hipStream_t st, Prepared* P);
+// Rebases a pre-summed call's workspace offsets onto `ws`.
+void bind_workspace(Prepared* P, void* ws);
+// Launches a prepared, bound call on its stream.
+int launch_prepared(const Prepared* P);
+// The shared classical (1x1x1, R = 1) algorithm.
+fmmhip_alg_t classical_alg();
+
+#endif

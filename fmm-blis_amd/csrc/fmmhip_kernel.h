@@ -274,26 +274,22 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     tn = in / gsz;
 }
 
+// One BM x BN tile position (tm, tn) of the leaf, all R products: the
+// body of both the one-problem kernel and the batched persistent kernel.
+// sA/sB are the workgroup's double-buffered LDS tiles.
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
-__global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>::NT),
-                          amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
-void fmm_gemm_kernel(KArgs a)
+__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*sA)[LDS_TILE], TL (*sB)[LDS_TILE])
 {
     constexpr int WAVES_M = Cfg<SHAPE>::WAVES_M;
     constexpr bool MULTI = Cfg<SHAPE>::MULTI;
     constexpr int WTM = Geo<SHAPE>::WTM, WTN = Geo<SHAPE>::WTN, FI = Geo<SHAPE>::FI, FJ = Geo<SHAPE>::FJ;
     typedef typename Leaf<TL>::acc_t acc_t;
-    __shared__ TL sA[2][LDS_TILE];
-    __shared__ TL sB[2][LDS_TILE];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = (wave % WAVES_M) * WTM;
     const int wn = (wave / WAVES_M) * WTN;
-
-    int tm, tn;
-    tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
 
     const TI* __restrict__ Ab = (const TI*)a.A;
@@ -317,10 +313,6 @@ void fmm_gemm_kernel(KArgs a)
     stA.init(tid, lda);
     stB.init(tid, ldb);
 
-    // epilogue coordinates of this lane's accumulator element (i, j, q):
-    // row = m0 + wm + 16 i + ROW_LANE (lane/16) + ROW_REG q, col = n0 + wn + 16 j + lane%16
-    const int er0 = wm + Leaf<TL>::ROW_LANE * (lane >> 4);
-    const int ec0 = wn + (lane & 15);
 
     for (int r = 0; r < R; r++) {
         const int u0 = uptr[r], nu = uptr[r + 1] - u0;
@@ -435,6 +427,10 @@ void fmm_gemm_kernel(KArgs a)
         // W epilogue: C_part += alpha*w*acc for every nonzero w of product r;
         // the part's first writer applies beta (beta == 0: C is not read).
         const int w0 = wptr[r], w1 = wptr[r + 1];
+        // epilogue coordinates of this lane's accumulator element (i, j, q):
+        // row = m0 + wm + 16 i + ROW_LANE (lane/16) + ROW_REG q, col = n0 + wn + 16 j + lane%16
+        const int er0 = wm + Leaf<TL>::ROW_LANE * (lane >> 4);
+        const int ec0 = wn + (lane & 15);
         for (int e = w0; e < w1; e++) {
             const fmm_entry_t we = ent[e];
             const double lam = a.alpha * we.coef;
@@ -465,6 +461,55 @@ void fmm_gemm_kernel(KArgs a)
         }
     }
 }
+
+template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
+__global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>::NT),
+                          amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
+void fmm_gemm_kernel(KArgs a)
+{
+    __shared__ TL sA[2][LDS_TILE];
+    __shared__ TL sB[2][LDS_TILE];
+    int tm, tn;
+    tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL>(a, tm, tn, sA, sB);
+}
+
+// Batched persistent kernel: a fixed grid of workgroups takes work items
+// (problem, tile) from a device queue until it is empty.  Items are listed
+// most expensive first; the queue head is one counter, reset by
+// batch_reset_kernel before each launch.
+struct BatchItem {
+    int problem;
+    int tile;                    // linear tile index in the problem's leaf grid
+};
+
+template <typename TI, typename TL, typename TO, int SHAPE, bool VEC>
+__global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>::NT),
+                          amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
+void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items, int nitems,
+                      int* __restrict__ head)
+{
+    __shared__ TL sA[2][LDS_TILE];
+    __shared__ TL sB[2][LDS_TILE];
+    __shared__ int slot;
+    for (;;) {
+        if (threadIdx.x == 0) slot = atomicAdd(head, 1);
+        __syncthreads();
+        // uniform across the workgroup: readfirstlane keeps the item and its
+        // problem descriptor in scalar registers
+        const int w = __builtin_amdgcn_readfirstlane(slot);
+        __syncthreads();                       // everyone has read slot before it is rewritten
+        if (w >= nitems) break;
+        const BatchItem it = items[w];
+        const KArgs a = probs[it.problem];
+        const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
+        gemm_tile<TI, TL, TO, SHAPE, VEC, true, false, true>(a, tm, tn, sA, sB);
+        __syncthreads();                       // LDS free before the next item's prologue
+    }
+}
+
+template <int UNUSED>   // a template, so each translation unit may instantiate it
+__global__ void batch_reset_kernel(int* head) { *head = 0; }
 
 // Turns runtime layout flags into template arguments.
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC>
@@ -501,6 +546,22 @@ int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, i
     return launch_layouts<TI, TL, TO, 0, false>(ka, a_mn, b_mn, c_col, nwg, st);
 }
 
+// Batched launch (column-major A, B, C): reset the queue head, then the
+// persistent kernel on `nwg` workgroups.
+template <typename TI, typename TL, typename TO>
+int launch_batch_typed(const KArgs* probs, const BatchItem* items, int nitems, int* head, bool vec, int nwg,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_reset_kernel<0>, dim3(1), dim3(1), 0, st, head);
+    if (vec)
+        hipLaunchKernelGGL((fmm_batch_kernel<TI, TL, TO, 0, true>), dim3(nwg), dim3(Cfg<0>::NT), 0, st, probs, items,
+                           nitems, head);
+    else
+        hipLaunchKernelGGL((fmm_batch_kernel<TI, TL, TO, 0, false>), dim3(nwg), dim3(Cfg<0>::NT), 0, st, probs, items,
+                           nitems, head);
+    return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+}
+
 }  // namespace fmmk
 
 // Per-datatype launchers (one translation unit each).
@@ -508,5 +569,9 @@ int fmmk_launch_d64(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool
 int fmmk_launch_d_f32leaf(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg,
                           hipStream_t st);
 int fmmk_launch_s32(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
+int fmmk_launch_batch_d64(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
+                          int nwg, hipStream_t st);
+int fmmk_launch_batch_s32(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
+                          int nwg, hipStream_t st);
 
 #endif

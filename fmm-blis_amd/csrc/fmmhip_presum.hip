@@ -48,7 +48,63 @@ __global__ __launch_bounds__(256) void presum_kernel(const T* __restrict__ X, lo
     }
 }
 
+// Batched form: segment s = one operand of one problem; blocks
+// [boff[s], boff[s+1]) process it, each block a contiguous range of pairs.
+template <typename T>
+__global__ __launch_bounds__(256) void presum_batch_kernel(const fmmhip_presum_seg_t* __restrict__ segs, int nseg,
+                                                           const int* __restrict__ boff)
+{
+    // the block's segment: binary search over the block offsets (uniform)
+    int lo = 0, hi = nseg - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boff[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const fmmhip_presum_seg_t sg = segs[lo];
+    const T* __restrict__ X = (const T*)sg.X;
+    T* __restrict__ S = (T*)sg.S;
+    const long pairs_c = (sg.Lc + 1) / 2;
+    const long total = pairs_c * sg.Ls;
+    const long plane = (long)sg.ldo * sg.Ls;
+    const long per_block = (total + (boff[lo + 1] - boff[lo]) - 1) / (boff[lo + 1] - boff[lo]);
+    const long i0 = (long)(b - boff[lo]) * per_block;
+    const long i1 = i0 + per_block < total ? i0 + per_block : total;
+    for (long idx = i0 + threadIdx.x; idx < i1; idx += blockDim.x) {
+        const int s = (int)(idx / pairs_c);
+        const int c = 2 * (int)(idx - (long)s * pairs_c);
+        for (int r = 0; r < sg.R; r++) {
+            double v0 = 0.0, v1 = 0.0;
+            for (int e = sg.ptr[r]; e < sg.ptr[r + 1]; e++) {
+                const fmm_entry_t en = sg.ent[e];
+                const int ec = sg.swap ? en.cols : en.rows;
+                const int es = sg.swap ? en.rows : en.cols;
+                if (s >= es) continue;
+                const T* p = X + en.off + c + (long)s * sg.ld;
+                if (c < ec) v0 = fma(en.coef, (double)p[0], v0);
+                if (c + 1 < ec) v1 = fma(en.coef, (double)p[1], v1);
+            }
+            T* q = S + r * plane + c + (long)s * sg.ldo;
+            q[0] = (T)v0;
+            if (c + 1 < sg.Lc) q[1] = (T)v1;
+        }
+    }
+}
+
 }  // namespace
+
+extern "C" int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff,
+                                   int nblocks, void* stream)
+{
+    if (nseg <= 0 || nblocks <= 0) return FMMHIP_SUCCESS;
+    if (is_double)
+        hipLaunchKernelGGL(presum_batch_kernel<double>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, segs, nseg,
+                           boff);
+    else
+        hipLaunchKernelGGL(presum_batch_kernel<float>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, segs, nseg,
+                           boff);
+    return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+}
 
 // ptr: CSR of the operand's entries over products (R+1 values, indexing ent).
 // swap: the entries' (rows, cols) are (s, c) rather than (c, s).

@@ -109,6 +109,20 @@ _SIGS = {
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm": (_I, [_L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
+    "fmmhip_batch_create": (_I, [ctypes.POINTER(_P)]),
+    "fmmhip_batch_destroy": (None, [_P]),
+    "fmmhip_batch_add_d": (_I, [_P, _L, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _I]),
+    "fmmhip_batch_add_s": (_I, [_P, _L, _L, _L, _F, _P, _L, _P, _L, _F, _P, _L, _I]),
+    "fmmhip_batch_plan": (_I, [_P]),
+    "fmmhip_batch_run": (_I, [_P, _P]),
+    "fmmhip_batch_info": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.POINTER(_L),
+                               ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_size_t)]),
+    "fmmhip_batch_choice": (_I, [_P, _L, ctypes.POINTER(_I)]),
+    "fmmhip_tuner_choose": (_I, [_I, _L, _L, _L, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
+    "fmmhip_tuner_estimate": (_I, [_I, _I, _L, _L, _L, _I, ctypes.POINTER(_D)]),
+    "fmmhip_tuner_calibrate": (_I, [_P]),
+    "fmmhip_tuner_save": (_I, [ctypes.c_char_p]),
+    "fmmhip_tuner_load": (_I, [ctypes.c_char_p]),
     "fmmhip_malloc": (_I, [ctypes.POINTER(_P), ctypes.c_size_t]),
     "fmmhip_free": (_I, [_P]),
     "fmmhip_memcpy_h2d": (_I, [_P, _P, ctypes.c_size_t]),
@@ -304,6 +318,86 @@ def dgemm(m, n, k, alpha, A, B, beta, C, stream=None):
 def sgemm(m, n, k, alpha, A, B, beta, C, stream=None):
     check(lib().fmmhip_sgemm(m, n, k, alpha, A.ptr, 1, A.ld, B.ptr, 1, B.ld, beta, C.ptr, 1, C.ld, stream),
           "sgemm")
+
+
+CAND_CLASSICAL, CAND_STRASSEN1, CAND_STRASSEN2 = 0, 1, 2
+
+
+class Batch:
+    """Many independent column-major GEMMs captured into one HIP graph
+    (fmmhip_batch_*).  add() takes DeviceMatrix operands of dtype f8
+    (FP64) or f4 (FP32); cand -1 lets the tuner choose the algorithm."""
+
+    def __init__(self):
+        self.h = _P()
+        check(lib().fmmhip_batch_create(ctypes.byref(self.h)), "batch_create")
+        self._keep = []
+
+    def add(self, m, n, k, alpha, A, B, beta, C, cand=-1):
+        if not (A.dtype == B.dtype == C.dtype):
+            raise FmmError("batch_add: operands of one problem share one datatype")
+        if C.dtype.itemsize == 8:
+            rc = lib().fmmhip_batch_add_d(self.h, m, n, k, alpha, A.ptr, A.ld, B.ptr, B.ld, beta, C.ptr, C.ld, cand)
+        else:
+            rc = lib().fmmhip_batch_add_s(self.h, m, n, k, alpha, A.ptr, A.ld, B.ptr, B.ld, beta, C.ptr, C.ld, cand)
+        check(rc, "batch_add")
+        self._keep.append((A, B, C))
+
+    def plan(self):
+        check(lib().fmmhip_batch_plan(self.h), "batch_plan")
+
+    def run(self, stream=None):
+        check(lib().fmmhip_batch_run(self.h, stream), "batch_run")
+
+    def info(self):
+        n, f64, f32, est, ws = _L(), _L(), _L(), _D(), ctypes.c_size_t()
+        check(lib().fmmhip_batch_info(self.h, ctypes.byref(n), ctypes.byref(f64), ctypes.byref(f32),
+                                      ctypes.byref(est), ctypes.byref(ws)), "batch_info")
+        return {"problems": n.value, "items_f64": f64.value, "items_f32": f32.value, "est_ms": est.value,
+                "workspace_bytes": ws.value}
+
+    def choice(self, i):
+        c = _I()
+        check(lib().fmmhip_batch_choice(self.h, i, ctypes.byref(c)), "batch_choice")
+        return c.value
+
+    def close(self):
+        if self.h:
+            lib().fmmhip_batch_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tuner_choose(is_double, m, n, k, lone=True):
+    """(candidate, estimated ms) the tuner picks for one problem."""
+    c, est = _I(), _D()
+    check(lib().fmmhip_tuner_choose(1 if is_double else 0, m, n, k, 1 if lone else 0, ctypes.byref(c),
+                                    ctypes.byref(est)), "tuner_choose")
+    return c.value, est.value
+
+
+def tuner_estimate(is_double, cand, m, n, k, lone=True):
+    est = _D()
+    check(lib().fmmhip_tuner_estimate(1 if is_double else 0, cand, m, n, k, 1 if lone else 0, ctypes.byref(est)),
+          "tuner_estimate")
+    return est.value
+
+
+def tuner_calibrate(stream=None):
+    check(lib().fmmhip_tuner_calibrate(stream), "tuner_calibrate")
+
+
+def tuner_save(path):
+    check(lib().fmmhip_tuner_save(path.encode()), "tuner_save")
+
+
+def tuner_load(path):
+    check(lib().fmmhip_tuner_load(path.encode()), "tuner_load")
 
 
 def fro_diff(X, Y):

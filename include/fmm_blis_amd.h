@@ -256,6 +256,41 @@ int fmmhip_sgemm(long m, long n, long k, float alpha,
                  const float* B, long rsB, long csB, float beta,
                  float* C, long rsC, long csC, void* stream);
 
+/* ---- batching form: many independent column-major GEMMs, one graph ---- */
+
+/* Candidates the tuner chooses from (also accepted as a forced choice):
+ * 0 classical, 1 one-level Strassen, 2 two-level Strassen (pre-summed). */
+typedef struct fmmhip_batch_s* fmmhip_batch_t;
+int  fmmhip_batch_create(fmmhip_batch_t* out);
+void fmmhip_batch_destroy(fmmhip_batch_t b);
+/* Adds C := beta*C + alpha*A*B (column-major, leading dimensions lda, ldb,
+ * ldc; device pointers).  cand = -1 lets the tuner choose.  Invalidates a
+ * previous plan. */
+int  fmmhip_batch_add_d(fmmhip_batch_t b, long m, long n, long k, double alpha, const double* A, long lda,
+                        const double* B, long ldb, double beta, double* C, long ldc, int cand);
+int  fmmhip_batch_add_s(fmmhip_batch_t b, long m, long n, long k, float alpha, const float* A, long lda,
+                        const float* B, long ldb, float beta, float* C, long ldc, int cand);
+/* Chooses every problem's candidate, prepares descriptors and the
+ * workspace, and captures one HIP graph: per datatype one batched
+ * summation launch and one persistent MFMA launch over (problem, tile)
+ * items, costliest first, taken from a device queue. */
+int  fmmhip_batch_plan(fmmhip_batch_t b);
+/* Launches the captured graph on `stream` (asynchronous; may be replayed). */
+int  fmmhip_batch_run(fmmhip_batch_t b, void* stream);
+int  fmmhip_batch_info(fmmhip_batch_t b, long* nprob, long* items_f64, long* items_f32, double* est_ms,
+                       size_t* ws_bytes);
+int  fmmhip_batch_choice(fmmhip_batch_t b, long i, int* cand);
+
+/* Tuner: the cost model's choice (and estimate, ms) for one problem, alone
+ * on the device (lone = 1) or inside a batch; the estimate of a given
+ * candidate; calibration by timing each candidate on the device; the
+ * calibrated table saved to / loaded from a text file. */
+int fmmhip_tuner_choose(int is_double, long m, long n, long k, int lone, int* cand, double* est_ms);
+int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, long k, int lone, double* est_ms);
+int fmmhip_tuner_calibrate(void* stream);
+int fmmhip_tuner_save(const char* path);
+int fmmhip_tuner_load(const char* path);
+
 /* Device memory, streams, events (thin wrappers so that C and ctypes
  * callers need no HIP headers). */
 int fmmhip_malloc(void** p, size_t bytes);

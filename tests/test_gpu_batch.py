@@ -1,0 +1,183 @@
+"""GPU parity of the batching form (fmmhip_batch_*): many uneven FP64 and
+FP32 GEMMs, each with its own algorithm (classical, 1- or 2-level Strassen,
+forced or chosen by the tuner), run as one captured HIP graph.
+
+Bars (stated per test):
+  * integer-valued inputs: every candidate is exact, so each problem's C
+    equals the oracle's classical product bit for bit, in both datatypes;
+  * uniform[-1,1] inputs: normalised error against the oracle below 5e-15
+    (FP64) and 1e-6 (FP32, against the product of the FP32-rounded inputs);
+  * a problem's result in the batch equals the same problem run alone
+    through the single-call entry with the same candidate, bit for bit;
+  * every replay of the graph, with eager work between replays, reproduces
+    the first replay bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fmm_blis_amd as F
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (dtype, m, n, k, cand); cand -1 = tuner's choice
+MIX = [
+    ("f8", 300, 280, 260, 2), ("f4", 300, 280, 260, 2), ("f8", 257, 255, 129, 1), ("f4", 257, 255, 129, 1),
+    ("f8", 64, 64, 64, 0), ("f4", 65, 63, 62, 0), ("f8", 1, 5, 3, 2), ("f4", 17, 1, 19, 1),
+    ("f8", 513, 100, 77, -1), ("f4", 100, 513, 77, -1), ("f8", 130, 140, 600, 2), ("f4", 130, 140, 600, 2),
+    ("f8", 31, 33, 35, 1), ("f4", 31, 33, 35, 0), ("f8", 384, 256, 128, 0), ("f4", 256, 384, 128, 1),
+    ("f8", 200, 64, 333, -1), ("f4", 96, 96, 1, 2), ("f8", 97, 101, 103, 2), ("f4", 7, 9, 11, -1),
+]
+ALPHA_BETA = [(1.0, 1.0), (2.0, -1.0), (1.0, 0.0), (-1.0, 2.0)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    rc = F.lib().fmmhip_init()
+    if rc != 0:
+        pytest.fail(f"no usable gfx950 device: {F.STATUS.get(rc)}")
+    yield
+
+
+def dev(X):
+    D = F.DeviceMatrix(X.shape[0], X.shape[1], dtype=X.dtype)
+    D.upload(X)
+    return D
+
+
+def inputs(dt, m, n, k, i, integer):
+    if integer:
+        amp = 4 if dt == "f8" else 2
+        A, B, C = O.integers(m, k, 11, 3 * i, amp), O.integers(k, n, 11, 3 * i + 1, amp), \
+            O.integers(m, n, 11, 3 * i + 2, amp)
+    else:
+        A, B, C = O.uniform(m, k, 12, 3 * i), O.uniform(k, n, 12, 3 * i + 1), O.uniform(m, n, 12, 3 * i + 2)
+    if dt == "f4":
+        A, B, C = (np.asfortranarray(X.astype(np.float32)) for X in (A, B, C))
+    return A, B, C
+
+
+def build(integer):
+    """Device operands and the batch over MIX; returns (batch, problems)."""
+    b = F.Batch()
+    probs = []
+    for i, (dt, m, n, k, cand) in enumerate(MIX):
+        A, B, C = inputs(dt, m, n, k, i, integer)
+        alpha, beta = ALPHA_BETA[i % len(ALPHA_BETA)]
+        dA, dB, dC = dev(A), dev(B), dev(C)
+        b.add(m, n, k, alpha, dA, dB, beta, dC, cand=cand)
+        probs.append(dict(dt=dt, m=m, n=n, k=k, cand=cand, alpha=alpha, beta=beta, A=A, B=B, C=C,
+                          dA=dA, dB=dB, dC=dC))
+    return b, probs
+
+
+def expected(p):
+    A, B, C = (np.asfortranarray(X.astype(np.float64)) for X in (p["A"], p["B"], p["C"]))
+    return O.dgemm(p["alpha"], A, B, p["beta"], C.copy(order="F"))
+
+
+def alg_for(cand):
+    if cand == 0:
+        return F.Alg(classical=True)
+    a = F.Alg([F.strassen()] * cand)
+    a.set_pack(1)
+    return a
+
+
+def test_batch_exact_integers_mixed_dtypes():
+    b, probs = build(integer=True)
+    b.plan()
+    info = b.info()
+    say("batch int", info)
+    assert info["problems"] == len(MIX)
+    assert info["items_f64"] > 0 and info["items_f32"] > 0
+    b.run()
+    F.sync()
+    for i, p in enumerate(probs):
+        got = p["dC"].to_numpy()
+        want = expected(p)
+        if p["dt"] == "f4":
+            want = want.astype(np.float32)
+        chosen = b.choice(i)
+        assert chosen in (0, 1, 2)
+        if p["cand"] >= 0:
+            assert chosen == p["cand"]
+        assert np.array_equal(got, want), (i, p["dt"], p["m"], p["n"], p["k"], chosen)
+
+
+def test_batch_random_error_and_alone_equals_mix():
+    b, probs = build(integer=False)
+    b.plan()
+    b.run()
+    F.sync()
+    for i, p in enumerate(probs):
+        got = p["dC"].to_numpy()
+        A64, B64 = p["A"].astype(np.float64), p["B"].astype(np.float64)
+        err = O.normalized_error(got.astype(np.float64), expected(p), A64, B64)
+        tol = 5e-15 if p["dt"] == "f8" else 1e-6
+        assert err < tol, (i, p["dt"], err)
+        # the same problem alone, same candidate, fresh C
+        cand = b.choice(i)
+        m, n, k = p["m"], p["n"], p["k"]
+        dC1 = dev(p["C"])
+        alg = alg_for(cand)
+        if p["dt"] == "f8":
+            F.dgemm_fmm(alg, m, n, k, p["alpha"], p["dA"], p["dB"], p["beta"], dC1)
+        else:
+            F.sgemm_fmm(alg, m, n, k, p["alpha"], p["dA"], p["dB"], p["beta"], dC1)
+        F.sync()
+        assert np.array_equal(dC1.to_numpy(), got), (i, p["dt"], m, n, k, cand)
+
+
+def test_batch_graph_replays_with_eager_work_between():
+    b, probs = build(integer=False)
+    b.plan()
+    first = None
+    side_A = F.DeviceMatrix(256, 256).fill_uniform(5, 0)
+    side_C = F.DeviceMatrix(256, 256).fill_uniform(5, 1)
+    for rep in range(3):
+        for p in probs:                       # reset every C eagerly
+            p["dC"].upload(p["C"])
+        F.dgemm(256, 256, 256, 1.0, side_A, side_A, 1.0, side_C)
+        b.run()
+        F.sync()
+        outs = [p["dC"].to_numpy() for p in probs]
+        say("replay", rep)
+        if first is None:
+            first = outs
+        else:
+            for i, (x, y) in enumerate(zip(first, outs)):
+                assert np.array_equal(x, y), (rep, i)
+
+
+def test_batch_tuner_candidates_and_replan():
+    """Large problems let the tuner pick Strassen; adding a problem drops the
+    plan (run refuses until re-planned)."""
+    b = F.Batch()
+    n = 2048
+    A = F.DeviceMatrix(n, n).fill_int(2, 0, 2)
+    Bm = F.DeviceMatrix(n, n).fill_int(2, 1, 2)
+    C1 = F.DeviceMatrix(n, n).fill_int(2, 2, 2)
+    C2 = F.DeviceMatrix(n, n).fill_int(2, 2, 2)
+    b.add(n, n, n, 1.0, A, Bm, 1.0, C1, cand=-1)
+    b.plan()
+    say("tuner choice at 2048^3 FP64:", b.choice(0))
+    b.run()
+    F.dgemm(n, n, n, 1.0, A, Bm, 1.0, C2)
+    F.sync()
+    assert F.fro_diff(C1, C2) == 0.0
+    s = F.DeviceMatrix(8, 8).fill_int(2, 3, 2)
+    b.add(8, 8, 8, 1.0, s, s, 0.0, F.DeviceMatrix(8, 8), cand=0)
+    with pytest.raises(F.FmmError):
+        b.run()
+    b.plan()
+    b.run()
+    F.sync()
+    b.close()

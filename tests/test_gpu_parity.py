@@ -103,7 +103,7 @@ def test_classical_dgemm_random(m, n, k):
 
 
 FMM_SHAPES = [(3, 13, 7), (31, 33, 35), (64, 64, 64), (97, 101, 103), (258, 260, 262), (1, 5, 3), (17, 1, 19),
-              (300, 280, 36)]
+              (300, 280, 36), (96, 96, 1)]
 
 
 @pytest.mark.parametrize("name", ALG_NAMES)

@@ -191,3 +191,58 @@ def test_c_header_layout_matches_binding(tmp_path):
     want = [ctypes.sizeof(F.ObjT), F.ObjT.buffer.offset, F.ObjT.rs.offset, F.ObjT.scalar.offset,
             F.ObjT.n_panel.offset, ctypes.sizeof(F.FmmT), F.FmmT.k_tilde.offset, F.FmmT.U.offset]
     assert got == want
+
+
+def test_tuner_choices_follow_the_cost_model():
+    """The tuner's choices (host logic only): classical for small or thin
+    problems, two-level Strassen for FP64 16384^3; a level needs leaves of
+    at least two tiles and k of at least 8 BK-steps per level."""
+    assert F.tuner_choose(True, 256, 256, 256)[0] == 0
+    assert F.tuner_choose(True, 16384, 16384, 16384)[0] == 2
+    assert F.tuner_choose(True, 16384, 16384, 64)[0] == 0           # rank-64 update
+    assert F.tuner_choose(True, 16384, 16384, 300)[0] in (0, 1)     # k too short for two levels
+    c, est = F.tuner_choose(False, 8192, 8192, 8192)
+    assert c in (0, 1, 2) and est > 0
+    # the chosen candidate's estimate is reported, and an FMM candidate is
+    # chosen only when it beats every simpler one by the 5% margin
+    for dbl in (True, False):
+        for (m, n, k) in [(4096, 4096, 4096), (16384, 16384, 16384), (2048, 8192, 1024)]:
+            c, est = F.tuner_choose(dbl, m, n, k)
+            assert abs(F.tuner_estimate(dbl, c, m, n, k) - est) < 1e-9
+            for simpler in range(c):
+                assert est < 0.95 * F.tuner_estimate(dbl, simpler, m, n, k) + 1e-12
+    # more work never costs less
+    assert F.tuner_estimate(True, 0, 8192, 8192, 8192) > F.tuner_estimate(True, 0, 4096, 4096, 4096)
+    with pytest.raises(F.FmmError):
+        F.tuner_choose(True, 0, 5, 5)
+
+
+def test_tuner_table_save_load_round_trip(tmp_path):
+    p = tmp_path / "tuner.txt"
+    F.tuner_save(str(p))
+    before = [F.tuner_estimate(True, c, 8192, 8192, 8192) for c in range(3)]
+    text = p.read_text()
+    assert text.count("cand ") == 6 and "bw " in text
+    # a table with another FP64 classical efficiency changes that estimate only
+    lines = [ln if not ln.startswith("cand 0 f64") else "cand 0 f64 0.390000 4.000000" for ln in text.splitlines()]
+    q = tmp_path / "t2.txt"
+    q.write_text("\n".join(lines) + "\n")
+    F.tuner_load(str(q))
+    try:
+        after = [F.tuner_estimate(True, c, 8192, 8192, 8192) for c in range(3)]
+        assert after[0] > 1.9 * before[0] and after[1:] == before[1:]
+    finally:
+        F.tuner_load(str(p))
+    assert [F.tuner_estimate(True, c, 8192, 8192, 8192) for c in range(3)] == before
+    bad = tmp_path / "bad.txt"
+    bad.write_text("nothing useful\n")
+    with pytest.raises(F.FmmError):
+        F.tuner_load(str(bad))
+    with pytest.raises(F.FmmError):
+        F.tuner_load(str(tmp_path / "missing.txt"))
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the behaviour without a device")
+def test_batch_create_without_device_fails():
+    h = ctypes.c_void_p()
+    assert F.lib().fmmhip_batch_create(ctypes.byref(h)) == 5       # FMMHIP_ERR_NODEVICE
