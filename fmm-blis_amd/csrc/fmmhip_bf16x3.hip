@@ -1,0 +1,497 @@
+// fmmhip_bf16x3.hip -- the BF16x3 leaf of the FP64 FMM (and classical) GEMM.
+//
+// Leaf products run on the bf16 MFMA with FP32 accumulation; operand
+// summation, the recombination of split terms and the W write-back into C
+// stay FP64.  Two stages per call:
+//
+//  1. split (HBM-bound): for every product r, the U- (V-) weighted sum of
+//     the operand's parts is formed in FP64 (as the pre-summed pack mode
+//     does) and written as three bf16 planes with one power-of-two scale per
+//     (row, 256-wide k chunk):
+//         x = 2^e * (d0 + (d1 + s2) 2^-9),
+//     e chosen so that |x| 2^-e < 256 over the chunk row; d0 and d1 are
+//     integers in [-256, 256] (exact in bf16), s2 = x 2^(9-e) - 512 d0 - d1
+//     (|s2| <= 1/2) rounded to bf16 -- the third 9-bit digit, pre-scaled by
+//     2^-9.  Representation error <= 2^-27 * (chunk row max).
+//  2. GEMM: one workgroup per 128 x 128 leaf tile of C, all R products; per
+//     32-wide k step, the plane products on v_mfma_f32_16x16x32_bf16 into
+//     three FP32 accumulators,
+//         G0 = sum d0 d0',
+//         G1 = sum d0 d1' + d1 d0' + d0 s2' + s2 d0',
+//         G2 = sum d1 d1'   (+ d1 s2' + s2 d1' with eight terms).
+//     G0 is a sum of at most 256 integer products of magnitude <= 2^16, so
+//     it is exact in FP32 (the rounding of G1 and G2 weighs 2^-33 and less);
+//     at every chunk end the three are recombined in FP64,
+//         acc += 2^(eA_i + eB_j) * (G0 + 2^-9 (G1 + 2^-9 G2)),
+//     and after the last chunk the W epilogue scatters alpha*w*acc into the
+//     C parts (race-free: the workgroup owns the tile in every part).
+//     Six terms drop the products of weight 2^-27 (d1 s2', s2 d1'); on
+//     uniform[-1,1] inputs that gives about 1.6x the normalised error of
+//     eight terms, whose error is that of the representation.
+//
+// Operand planes in HBM: P[r][g][kb][row][32] (kb = k/32): one k step of a
+// 128-row tile is 8 KB contiguous.  Rows are padded to a multiple of 128
+// and k to a multiple of 256 with zeros, so the GEMM loads need no bounds.
+// Exponents E[r][chunk][row].  Non-finite inputs make their chunk row NaN.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "fmmhip_host.h"
+
+namespace bf3 {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes, a native vector
+
+constexpr int KSTEP = 32;        // k per MFMA stage
+constexpr int CHUNK = 256;       // k per exact FP32 chunk
+constexpr int STEPS_PER_CHUNK = CHUNK / KSTEP;
+constexpr int ROW_PAD = 128;     // plane rows are padded to this
+constexpr int SROWS = 16;        // rows per split block
+
+// ---- split ------------------------------------------------------------------
+
+struct SplitArgs {
+    const double* X;
+    long sr, sk;                 // element (row i, k) of a part at X[off + i*sr + k*sk]
+    const int* ptr;              // CSR of this operand's entries over products
+    const fmm_entry_t* ent;
+    int swap;                    // entries' rows are the k extent (operand B)
+    int Lp, ldk;                 // padded rows, padded k
+    u16* P;                      // planes
+    int* E;                      // exponents
+};
+
+__device__ __forceinline__ u16 bf16_exact(double d)   // |d| integer <= 256
+{
+    return (u16)(__float_as_uint((float)d) >> 16);
+}
+
+__device__ __forceinline__ u16 bf16_rne(double d)
+{
+    uint32_t u = __float_as_uint((float)d);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+
+// Block: SROWS rows x one CHUNK of k of product blockIdx.z.
+template <bool ROWC>
+__global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
+{
+    __shared__ __attribute__((aligned(16))) double T[SROWS][CHUNK + 1];
+    const int r = blockIdx.z;
+    const int row0 = blockIdx.x * SROWS;
+    const int k0 = blockIdx.y * CHUNK;
+    const int t = threadIdx.x;
+    const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
+    // 1. weighted sum of the parts, zero outside their extents
+#pragma unroll 4
+    for (int i = 0; i < SROWS * CHUNK / 256; i++) {
+        const int idx = t + 256 * i;
+        const int row = ROWC ? (idx % SROWS) : (idx / CHUNK);
+        const int kk = ROWC ? (idx / SROWS) : (idx % CHUNK);
+        const int gi = row0 + row, gk = k0 + kk;
+        double v = 0.0;
+        for (int e = e0; e < e1; e++) {
+            const fmm_entry_t en = a.ent[e];
+            const int er = a.swap ? en.cols : en.rows;
+            const int ek = a.swap ? en.rows : en.cols;
+            if (gi < er && gk < ek) v = fma(en.coef, a.X[en.off + (long)gi * a.sr + (long)gk * a.sk], v);
+        }
+        T[row][kk] = v;
+    }
+    __syncthreads();
+    // 2. per row: scale from the chunk row max, then the three digits
+    const int row = t >> 4, s = t & 15;
+    double x[CHUNK / 16];
+    double mx = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < CHUNK / 16; j++) {
+        x[j] = T[row][s + 16 * j];
+        mx = fmax(mx, fabs(x[j]));
+        bad |= !isfinite(x[j]);
+    }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        bad |= __shfl_xor(bad, o);
+    }
+    int E = 0;
+    if (!bad && mx > 0.0) (void)frexp(mx, &E);      // mx < 2^E
+    const int esh = 8 - E;                           // |x| 2^esh < 256
+    __syncthreads();                                 // T is reused for the planes
+    u16* O = (u16*)&T[0][0];                         // O[g][row][CHUNK]
+#pragma unroll
+    for (int j = 0; j < CHUNK / 16; j++) {
+        const int kk = s + 16 * j;
+        u16 p0, p1, p2;
+        if (bad) {
+            p0 = 0x7FC0; p1 = 0; p2 = 0;
+        } else {
+            const double y = ldexp(x[j], esh);
+            const double d0 = rint(y);
+            const double r1 = (y - d0) * 512.0;
+            const double d1 = rint(r1);
+            const double s2 = r1 - d1;              // third digit times 2^-9
+            p0 = bf16_exact(d0);
+            p1 = bf16_exact(d1);
+            p2 = bf16_rne(s2);
+        }
+        O[(0 * SROWS + row) * CHUNK + kk] = p0;
+        O[(1 * SROWS + row) * CHUNK + kk] = p1;
+        O[(2 * SROWS + row) * CHUNK + kk] = p2;
+    }
+    if (s == 0) a.E[((long)r * (a.ldk / CHUNK) + blockIdx.y) * a.Lp + row0 + row] = bad ? 0 : -esh;
+    __syncthreads();
+    // 3. write out: per (plane, k step) SROWS rows of 64 B, contiguous
+    const long plane = (long)a.ldk * a.Lp;
+#pragma unroll
+    for (int i = 0; i < 3 * SROWS * CHUNK / 8 / 256; i++) {
+        const int idx = t + 256 * i;
+        const int g = idx / (SROWS * CHUNK / 8);
+        const int rem = idx % (SROWS * CHUNK / 8);
+        const int kbl = rem / (SROWS * 4);
+        const int rem2 = rem % (SROWS * 4);
+        const int rr = rem2 >> 2, c = rem2 & 3;
+        const uint4 v = *(const uint4*)&O[(g * SROWS + rr) * CHUNK + kbl * KSTEP + c * 8];
+        u16* dst = a.P + ((long)r * 3 + g) * plane + ((long)(k0 / KSTEP + kbl) * a.Lp + row0 + rr) * KSTEP + c * 8;
+        *(uint4*)dst = v;
+    }
+}
+
+// ---- GEMM -------------------------------------------------------------------
+
+struct GemmArgs {
+    const u16* A;
+    const int* EA;
+    const u16* B;
+    const int* EB;
+    long a_plane, b_plane;       // elements per plane
+    long ea_prod, eb_prod;       // exponents per product
+    int LpA, LpB;
+    int nkb;                     // k steps per product (multiple of STEPS_PER_CHUNK)
+    int R;
+    const int* wptr;             // W CSR over products (absolute entry indices)
+    const fmm_entry_t* ent;
+    double* C;
+    long rsC, csC;
+    double alpha, beta;
+    int tiles_m, tiles_n;
+    int min_cm, min_cn;          // tiles inside [0,min_cm) x [0,min_cn) are inside every C part
+};
+
+template <int WM, int WN, int FI, int FJ>
+struct GCfg {
+    static constexpr int NT = WM * WN * 64;
+    static constexpr int BM = WM * FI * 16, BN = WN * FJ * 16;
+    static constexpr int A_CH = 3 * BM * 4, B_CH = 3 * BN * 4;   // 16-byte chunks per stage
+    static constexpr int A_PER = A_CH / NT, B_PER = B_CH / NT;
+    static_assert((BM * 4) % NT == 0 && (BN * 4) % NT == 0, "a plane's chunks divide among threads");
+    static_assert(BM + BN <= NT, "one exponent per thread");
+};
+
+// chunk c (16 bytes = 8 k) of row `row` in an LDS stage: swizzled so that
+// the 16-byte reads of eight consecutive rows hit distinct banks
+__device__ __forceinline__ int lds_chunk(int row, int c) { return row * 4 + (c ^ ((row >> 1) & 3)); }
+
+template <int WM, int WN, int FI, int FJ, int TERMS>
+__global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
+{
+    typedef GCfg<WM, WN, FI, FJ> G;
+    constexpr int BM = G::BM, BN = G::BN, NT = G::NT;
+    __shared__ u32x4 sA[2][G::A_CH];
+    __shared__ u32x4 sB[2][G::B_CH];
+    __shared__ int sE[2][BM + BN];
+
+    int tm, tn;
+    fmmk::tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave % WM) * (FI * 16), wn = (wave / WM) * (FJ * 16);
+    const int S = a.R * a.nkb;
+
+    // LDS chunk of the thread's staging row/chunk (the row swizzle depends on tid only)
+    const int st_base = (tid >> 2) * 4 + ((tid & 3) ^ ((tid >> 3) & 3));   // lds_chunk(tid >> 2, tid & 3)
+    // One k step of the A and B planes in flight from HBM to LDS.  Thread
+    // tid moves 16-byte chunk tid + NT*i of every stage; with (rows*4) % NT
+    // == 0 the chunk's plane is i / P and its row (tid >> 2) + NT (i % P) / 4,
+    // so every address is a per-thread base plus a compile-time offset.  The
+    // staging arrays are plain locals indexed in unrolled loops (small enough
+    // to be kept in registers as vectors).
+    constexpr int PA = BM * 4 / NT, PB = BN * 4 / NT;
+    u32x4 ra[G::A_PER];
+    u32x4 rb[G::B_PER];
+    int re = 0;
+#define BF3_GLOAD(R_, KB_)                                                                                   \
+    do {                                                                                                     \
+        const int r_ = (R_), kb_ = (KB_);                                                                    \
+        const u16* Ab_ = a.A + (long)r_ * 3 * a.a_plane + ((long)kb_ * a.LpA + m0) * KSTEP + tid * 8;        \
+        const u16* Bb_ = a.B + (long)r_ * 3 * a.b_plane + ((long)kb_ * a.LpB + n0) * KSTEP + tid * 8;        \
+        _Pragma("unroll") for (int i_ = 0; i_ < G::A_PER; i_++)                                              \
+            ra[i_] = *(const u32x4*)(Ab_ + (i_ / PA) * a.a_plane + (i_ % PA) * NT * 8);                       \
+        _Pragma("unroll") for (int i_ = 0; i_ < G::B_PER; i_++)                                              \
+            rb[i_] = *(const u32x4*)(Bb_ + (i_ / PB) * a.b_plane + (i_ % PB) * NT * 8);                       \
+        if (kb_ % STEPS_PER_CHUNK == 0) {                                                                    \
+            const int ch_ = kb_ / STEPS_PER_CHUNK;                                                           \
+            if (tid < BM) re = a.EA[(long)r_ * a.ea_prod + (long)ch_ * a.LpA + m0 + tid];                    \
+            else if (tid < BM + BN) re = a.EB[(long)r_ * a.eb_prod + (long)ch_ * a.LpB + n0 + tid - BM];     \
+        }                                                                                                    \
+    } while (0)
+#define BF3_LSTORE(ST_, CHUNK_START_, EP_)                                                                   \
+    do {                                                                                                     \
+        _Pragma("unroll") for (int i_ = 0; i_ < G::A_PER; i_++)                                              \
+            sA[ST_][st_base + (i_ / PA) * BM * 4 + (i_ % PA) * NT] = ra[i_];                                 \
+        _Pragma("unroll") for (int i_ = 0; i_ < G::B_PER; i_++)                                              \
+            sB[ST_][st_base + (i_ / PB) * BN * 4 + (i_ % PB) * NT] = rb[i_];                                 \
+        if ((CHUNK_START_) && tid < BM + BN) sE[EP_][tid] = re;                                              \
+    } while (0)
+    floatx4 G0[FI][FJ], G1[FI][FJ], G2[FI][FJ];
+    double acc[FI][FJ][4];
+#pragma unroll
+    for (int i = 0; i < FI; i++)
+#pragma unroll
+        for (int j = 0; j < FJ; j++) {
+            G0[i][j] = G1[i][j] = G2[i][j] = (floatx4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[i][j][q] = 0.0;
+        }
+
+    BF3_GLOAD(0, 0);
+    BF3_LSTORE(0, true, 0);
+    __syncthreads();
+
+    // fragment reads: row wm + 16 i + (lane & 15) (resp. wn + 16 j + ...),
+    // 8-wide k block kc = lane >> 4; the swizzle depends on lane & 15 only
+    const int kc = lane >> 4;
+    const int rd_a = (wm + (lane & 15)) * 4 + (kc ^ (((lane & 15) >> 1) & 3));
+    const int rd_b = (wn + (lane & 15)) * 4 + (kc ^ (((lane & 15) >> 1) & 3));
+    int st = 0;           // LDS stage of the current step
+    int chunk = 0;        // chunks completed (exponent buffer parity)
+    for (int r = 0; r < a.R; r++) {
+        for (int kb = 0; kb < a.nkb; kb++) {
+            int rn = r, kbn = kb + 1;
+            if (kbn == a.nkb) { rn = r + 1; kbn = 0; }
+            const bool more = rn < a.R;
+            if (more) BF3_GLOAD(rn, kbn);
+
+            // the plane products of the step
+            const u32x4* la = sA[st];
+            const u32x4* lb = sB[st];
+            auto fa = [&](int g, int i) { return __builtin_bit_cast(bf16x8, la[rd_a + (g * BM + 16 * i) * 4]); };
+            auto fb = [&](int g, int j) { return __builtin_bit_cast(bf16x8, lb[rd_b + (g * BN + 16 * j) * 4]); };
+            {
+#define BF3_MFMA(ACC, X, Y) ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, Y, ACC, 0, 0, 0)
+                bf16x8 a0[FI], b0[FJ], a1[FI], b1[FJ];
+#pragma unroll
+                for (int i = 0; i < FI; i++) a0[i] = fa(0, i);
+#pragma unroll
+                for (int j = 0; j < FJ; j++) b0[j] = fb(0, j);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) BF3_MFMA(G0[i][j], a0[i], b0[j]);
+#pragma unroll
+                for (int j = 0; j < FJ; j++) b1[j] = fb(1, j);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) BF3_MFMA(G1[i][j], a0[i], b1[j]);
+#pragma unroll
+                for (int i = 0; i < FI; i++) a1[i] = fa(1, i);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) {
+                        BF3_MFMA(G1[i][j], a1[i], b0[j]);
+                        BF3_MFMA(G2[i][j], a1[i], b1[j]);
+                    }
+                bf16x8 b2[FJ];
+#pragma unroll
+                for (int j = 0; j < FJ; j++) b2[j] = fb(2, j);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) {
+                        BF3_MFMA(G1[i][j], a0[i], b2[j]);
+                        if (TERMS == 8) BF3_MFMA(G2[i][j], a1[i], b2[j]);
+                    }
+                bf16x8 a2[FI];
+#pragma unroll
+                for (int i = 0; i < FI; i++) a2[i] = fa(2, i);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) {
+                        BF3_MFMA(G1[i][j], a2[i], b0[j]);
+                        if (TERMS == 8) BF3_MFMA(G2[i][j], a2[i], b1[j]);
+                    }
+#undef BF3_MFMA
+            }
+
+            if (kb % STEPS_PER_CHUNK == STEPS_PER_CHUNK - 1) {
+                // chunk end: FP64 recombination with the chunk's row/column scales
+                const int* eA = sE[chunk & 1];
+                const int* eB = eA + BM;
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) {
+                        const int eb = eB[wn + 16 * j + (lane & 15)];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int ea = eA[wm + 16 * i + 4 * (lane >> 4) + q];
+                            const float g12 = fmaf(G2[i][j][q], 0x1p-9f, G1[i][j][q]);
+                            const double t = fma((double)g12, 0x1p-9, (double)G0[i][j][q]);
+                            acc[i][j][q] += ldexp(t, ea + eb);
+                        }
+                        G0[i][j] = G1[i][j] = G2[i][j] = (floatx4){0.f, 0.f, 0.f, 0.f};
+                    }
+                chunk++;
+            }
+            if (kb == a.nkb - 1) {
+                // product end: W epilogue, C_part += alpha*w*acc; the part's
+                // first writer applies beta (beta == 0: C is not read)
+                const bool c_in = m0 + BM <= a.min_cm && n0 + BN <= a.min_cn;
+                const int er0 = wm + 4 * (lane >> 4), ec0 = wn + (lane & 15);
+                for (int e = a.wptr[r]; e < a.wptr[r + 1]; e++) {
+                    const fmm_entry_t we = a.ent[e];
+                    const double lam = a.alpha * we.coef;
+                    const bool first = we.first != 0;
+                    double* ct = a.C + we.off + (long)(m0 + er0) * a.rsC + (long)(n0 + ec0) * a.csC;
+                    const int erows = we.rows - m0 - er0, ecols = we.cols - n0 - ec0;
+#pragma unroll
+                    for (int j = 0; j < FJ; j++)
+#pragma unroll
+                        for (int i = 0; i < FI; i++)
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                const int dr = 16 * i + q, dc = 16 * j;
+                                if (c_in || (dr < erows && dc < ecols)) {
+                                    double* p = ct + (long)dr * a.rsC + (long)dc * a.csC;
+                                    double old = 0.0;
+                                    if (!first) old = *p;
+                                    else if (a.beta != 0.0) old = a.beta * *p;
+                                    *p = fma(lam, acc[i][j][q], old);
+                                }
+                            }
+                }
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc[i][j][q] = 0.0;
+            }
+            if (more) BF3_LSTORE(st ^ 1, kbn % STEPS_PER_CHUNK == 0, chunk & 1);
+            st ^= 1;
+            __syncthreads();
+        }
+    }
+#undef BF3_GLOAD
+#undef BF3_LSTORE
+}
+
+}  // namespace bf3
+
+// ---- host ---------------------------------------------------------------------
+
+namespace {
+
+constexpr int kWM = 2, kWN = 4, kFI = 4, kFJ = 2;
+typedef bf3::GCfg<kWM, kWN, kFI, kFJ> Cfg3;
+
+long round_up(long x, long m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alpha, const double* A, long rsA, long csA,
+                const double* B, long rsB, long csB, double beta, double* C, long rsC, long csC, hipStream_t st)
+{
+    if (!alg || m < 0 || n < 0 || k < 0 || (terms != 6 && terms != 8)) return FMMHIP_ERR_INVALID;
+    if (m == 0 || n == 0) return FMMHIP_SUCCESS;
+    if (!C) return FMMHIP_ERR_INVALID;
+    if (k == 0 || alpha == 0.0) return fmmhip_scale_d(C, m, n, rsC, csC, beta, st);
+    if (!A || !B) return FMMHIP_ERR_INVALID;
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+
+    fmm_geom_t g;
+    memset(&g, 0, sizeof(g));
+    g.m = m; g.n = n; g.k = k;
+    g.lm = (m + alg->mt - 1) / alg->mt;
+    g.lk = (k + alg->kt - 1) / alg->kt;
+    g.ln = (n + alg->nt - 1) / alg->nt;
+    g.rsA = rsA; g.csA = csA; g.rsB = rsB; g.csB = csB; g.rsC = rsC; g.csC = csC;
+    const int* dptr = nullptr;
+    const fmm_entry_t* dent = nullptr;
+    int rc = fused_desc(alg, g, st, &dptr, &dent);
+    if (rc) return rc;
+
+    const int R = alg->R;
+    const long LpA = round_up(g.lm, bf3::ROW_PAD), LpB = round_up(g.ln, bf3::ROW_PAD);
+    const long ldk = round_up(g.lk, bf3::CHUNK);
+    const long nch = ldk / bf3::CHUNK;
+    if (LpA > INT32_MAX || LpB > INT32_MAX || ldk > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+    const size_t planesA = (size_t)R * 3 * LpA * ldk * 2, planesB = (size_t)R * 3 * LpB * ldk * 2;
+    const size_t expA = (size_t)R * nch * LpA * 4, expB = (size_t)R * nch * LpB * 4;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t oPB = al(planesA), oEA = oPB + al(planesB), oEB = oEA + al(expA);
+    const size_t bytes = oEB + al(expB);
+    char* ws = (char*)workspace_for(st, bytes);
+    if (!ws) return FMMHIP_ERR_NOMEM;
+
+    bf3::SplitArgs sa;
+    sa.X = A; sa.sr = rsA; sa.sk = csA; sa.ptr = dptr; sa.ent = dent; sa.swap = 0;
+    sa.Lp = (int)LpA; sa.ldk = (int)ldk; sa.P = (bf3::u16*)ws; sa.E = (int*)(ws + oEA);
+    bf3::SplitArgs sb;
+    sb.X = B; sb.sr = csB; sb.sk = rsB; sb.ptr = dptr + (R + 1); sb.ent = dent; sb.swap = 1;
+    sb.Lp = (int)LpB; sb.ldk = (int)ldk; sb.P = (bf3::u16*)(ws + oPB); sb.E = (int*)(ws + oEB);
+
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    const bool timed = timing_active();
+    if (timed) timing_record(&e0, st);
+    const dim3 blk(256);
+    const dim3 gA((unsigned)(LpA / bf3::SROWS), (unsigned)nch, (unsigned)R);
+    const dim3 gB((unsigned)(LpB / bf3::SROWS), (unsigned)nch, (unsigned)R);
+    if (rsA == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gA, blk, 0, st, sa);
+    else hipLaunchKernelGGL(bf3::split_kernel<false>, gA, blk, 0, st, sa);
+    if (csB == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gB, blk, 0, st, sb);
+    else hipLaunchKernelGGL(bf3::split_kernel<false>, gB, blk, 0, st, sb);
+    if (hipGetLastError() != hipSuccess) return FMMHIP_ERR_HIP;
+    if (timed) timing_record(&e1, st);
+
+    bf3::GemmArgs ga;
+    ga.A = (const bf3::u16*)ws; ga.EA = (const int*)(ws + oEA);
+    ga.B = (const bf3::u16*)(ws + oPB); ga.EB = (const int*)(ws + oEB);
+    ga.a_plane = LpA * ldk; ga.b_plane = LpB * ldk;
+    ga.ea_prod = nch * LpA; ga.eb_prod = nch * LpB;
+    ga.LpA = (int)LpA; ga.LpB = (int)LpB;
+    ga.nkb = (int)(ldk / bf3::KSTEP);
+    ga.R = R;
+    ga.wptr = dptr + 2 * (R + 1);
+    ga.ent = dent;
+    ga.C = C; ga.rsC = rsC; ga.csC = csC;
+    ga.alpha = alpha; ga.beta = beta;
+    ga.tiles_m = (int)(LpA / Cfg3::BM);
+    ga.tiles_n = (int)(LpB / Cfg3::BN);
+    ga.min_cm = (int)(m / alg->mt);
+    ga.min_cn = (int)(n / alg->nt);
+    const long nwg = (long)ga.tiles_m * ga.tiles_n;
+    if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+    if (terms == 8)
+        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 8>), dim3((unsigned)nwg), dim3(Cfg3::NT), 0,
+                           st, ga);
+    else
+        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 6>), dim3((unsigned)nwg), dim3(Cfg3::NT), 0,
+                           st, ga);
+    if (hipGetLastError() != hipSuccess) return FMMHIP_ERR_HIP;
+    if (timed) {
+        timing_record(&e2, st);
+        timing_push(e0, e1, e2, true);
+    }
+    return FMMHIP_SUCCESS;
+}

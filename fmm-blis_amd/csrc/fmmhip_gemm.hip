@@ -447,6 +447,32 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
     return launch_prepared(&P);
 }
 
+int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int** dptr, const fmm_entry_t** dent)
+{
+    DevDesc* d = nullptr;
+    int rc = get_desc(alg, g, false, true, false, st, &d);
+    if (rc) return rc;
+    *dptr = d->fused.dptr();
+    *dent = d->fused.dent();
+    return FMMHIP_SUCCESS;
+}
+
+void* workspace_for(hipStream_t st, size_t bytes) { return workspace_get(st, bytes); }
+
+bool timing_active()
+{
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    return g_tm_on;
+}
+
+void timing_record(hipEvent_t* ev, hipStream_t st) { tm_record(ev, st); }
+
+void timing_push(hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, bool presum)
+{
+    std::lock_guard<std::mutex> lk(g_tm_mu);
+    g_tm.push_back(TimedCall{e0, e1, e2, presum});
+}
+
 fmmhip_alg_t classical_alg()
 {
     static std::once_flag once;
@@ -537,6 +563,14 @@ extern "C" int fmmhip_dgemm_fmm(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, lo
     switch (leaf) {
     case FMMHIP_LEAF_NATIVE: kind = Kind::D64; break;
     case FMMHIP_LEAF_F32: kind = Kind::D_F32LEAF; break;
+    case FMMHIP_LEAF_BF16X3:
+    case FMMHIP_LEAF_BF16X3_6: {
+        int rc = fmmhip_init();
+        if (rc) return rc;
+        hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
+        return bf16x3_gemm(alg, leaf == FMMHIP_LEAF_BF16X3 ? 8 : 6, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta,
+                           C, rsC, csC, st);
+    }
     default: return FMMHIP_ERR_UNSUPPORTED;
     }
     return gemm_common(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, stream);

@@ -37,4 +37,21 @@ int launch_prepared(const Prepared* P);
 // The shared classical (1x1x1, R = 1) algorithm.
 fmmhip_alg_t classical_alg();
 
+// The cached fused descriptor of geometry g (device CSR U|V|W and entries).
+int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int** dptr,
+               const fmm_entry_t** dent);
+// The stream's grow-only workspace of at least `bytes` (nullptr: no memory).
+void* workspace_for(hipStream_t st, size_t bytes);
+// Launch timing hooks (fmmhip_timing_*): whether timing is on, an event
+// recorded on `st`, and one timed call (summation/split from e0 to e1,
+// MFMA kernel from e1 to e2).
+bool timing_active();
+void timing_record(hipEvent_t* ev, hipStream_t st);
+void timing_push(hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, bool presum);
+
+// The BF16x3 leaf (fmmhip_bf16x3.hip): split stage and bf16 MFMA GEMM with
+// `terms` (6 or 8) plane products.
+int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alpha, const double* A, long rsA, long csA,
+                const double* B, long rsB, long csB, double beta, double* C, long rsC, long csC, hipStream_t st);
+
 #endif

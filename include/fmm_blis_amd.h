@@ -150,8 +150,11 @@ void bli_strassen_ab(obj_t* alpha, obj_t* A, obj_t* B, obj_t* beta, obj_t* C);
 typedef enum {
     FMMHIP_LEAF_NATIVE = 0,   /* the operands' type: FP64 for double, FP32 for float */
     FMMHIP_LEAF_F32    = 1,   /* FP64 operands, FP32 MFMA leaf, FP64 writeback       */
-    FMMHIP_LEAF_BF16X3 = 2    /* FP64 operands, split BF16 MFMA leaf (FP32 accumulate),
-                                 FP64 recombination and writeback                     */
+    FMMHIP_LEAF_BF16X3 = 2,   /* FP64 operands, split BF16 MFMA leaf (FP32 accumulate),
+                                 FP64 recombination and writeback: three bf16 planes
+                                 per operand, eight plane products                    */
+    FMMHIP_LEAF_BF16X3_6 = 3  /* as BF16X3 with six plane products (faster, about
+                                 1.6x the error)                                      */
 } fmmhip_leaf_t;
 
 /* Like bli_strassen_ab_ex with `nlevels` levels; level l uses algs[l]
