@@ -194,6 +194,91 @@ def run_batch(args, F):
     tuned.close()
 
 
+def serve_stream(count, rate, seed, max_dim):
+    """(dtype, m, n, k, arrival offset s): m, n, k log-uniform in
+    [128, max_dim] rounded to multiples of 32, 70% FP64, Poisson arrivals."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += rng.exponential(1.0 / rate)
+        dims = [int(min(max_dim, max(128, round(float(np.exp(rng.uniform(np.log(128.0), np.log(max_dim)))) / 32.0) * 32)))
+                for _ in range(3)]
+        out.append(("f8" if rng.uniform() < 0.7 else "f4", dims[0], dims[1], dims[2], t))
+    return out
+
+
+def run_serve(args, F):
+    """The serving form: a Poisson stream of mixed FP64/FP32 GEMMs over
+    device-resident operand pools, admitted by HBM budget onto `slots`
+    streams by the library's request loop; latency from arrival to observed
+    completion."""
+    import ctypes
+    D = args.max_dim
+    pools = {"f8": (F.DeviceMatrix(D, D).fill_uniform(args.seed, 0), F.DeviceMatrix(D, D).fill_uniform(args.seed, 1)),
+             "f4": (F.DeviceMatrix(D, D, dtype="f4").fill_uniform(args.seed, 2),
+                    F.DeviceMatrix(D, D, dtype="f4").fill_uniform(args.seed, 3))}
+    F.sync()
+    free_b, tot_b = ctypes.c_size_t(), ctypes.c_size_t()
+    F.check(F.lib().fmmhip_mem_info(ctypes.byref(free_b), ctypes.byref(tot_b)), "mem")
+    budget = int(args.budget_gib * (1 << 30)) if args.budget_gib > 0 else int(min(0.85 * free_b.value, 256 << 30))
+
+    def run(srv, stream, t0):
+        for dt, m, n, k, t in stream:
+            A, B = pools[dt]
+            req = F.Request()
+            req.is_double = 1 if dt == "f8" else 0
+            req.m, req.n, req.k, req.alpha, req.beta = m, n, k, 1.0, 0.0
+            req.A, req.lda = A.ptr.value, A.ld
+            req.B, req.ldb = B.ptr.value, B.ld
+            req.C, req.ldc = None, 0
+            req.cand, req.keep, req.arrival_s = -1, 0, t0 + t
+            srv.submit_request(req)
+        srv.drain()
+        return srv.stats()
+
+    # warm-up stream (kernels loaded, tuner and arena paths exercised)
+    warm = F.Server(budget, slots=args.slots)
+    run(warm, serve_stream(max(args.warmup, 1) * 8, args.rate, args.seed + 1, min(D, 2048)), warm.now() + 0.01)
+    warm.close()
+
+    stream = serve_stream(args.requests, args.rate, args.seed, D)
+    srv = F.Server(budget, slots=args.slots)
+    t_wall = time.perf_counter()
+    st = run(srv, stream, srv.now() + 0.05)
+    wall = time.perf_counter() - t_wall
+    srv.close()
+    flops = sum(2.0 * m * n * k for (_, m, n, k, _) in stream)
+    flops64 = sum(2.0 * m * n * k for (dt, m, n, k, _) in stream if dt == "f8")
+    span = st["span_s"]
+    out = {
+        "metric": "serving form: p99 GEMM latency (arrival to completion) at a Poisson request rate",
+        "value": round(st["p99_ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1, "steps": args.requests, "warmup": max(args.warmup, 1) * 8,
+        "ms_per_step": round(span * 1e3 / max(st["completed"], 1), 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": f"synthetic (SplitMix64 uniform[-1,1] operand pools {D}x{D}, seed {args.seed}, device-resident)",
+        "config": {
+            "workload": f"{args.requests} GEMM requests C = A*B, Poisson arrivals at {args.rate:g}/s, m,n,k "
+                        f"log-uniform [128,{D}] multiples of 32, 70% FP64 / 30% FP32, {args.slots} streams, "
+                        f"HBM budget {budget / (1 << 30):.1f} GiB, algorithm per request by the tuner",
+            "rate_per_s": args.rate, "requests": args.requests, "slots": args.slots, "budget_bytes": budget,
+            "completed": st["completed"], "failed": st["failed"],
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_ms": round(st["mean_ms"], 3),
+            "max_ms": round(st["max_ms"], 3),
+            "throughput_req_per_s": round(st["completed"] / span, 2) if span > 0 else None,
+            "effective_gflops": round(flops / span / 1e9, 1) if span > 0 else None,
+            "fp64_flop_share": round(flops64 / flops, 4),
+            "span_s": round(span, 4), "wall_s": round(wall, 3),
+            "arena_peak_bytes": st["arena_peak"], "max_inflight": st["max_inflight"],
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,9 +293,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against the classical kernel")
     ap.add_argument("--profile", action="store_true")
-    ap.add_argument("--form", default="single", choices=["single", "batch"],
-                    help="single: one m=n=k product; batch: 512 uneven FP64/FP32 GEMMs in one graph")
+    ap.add_argument("--form", default="single", choices=["single", "batch", "serve"],
+                    help="single: one m=n=k product; batch: 512 uneven FP64/FP32 GEMMs in one graph; "
+                         "serve: Poisson request stream through the continuous-batching server")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--rate", type=float, default=500.0, help="serve: arrivals per second")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the timed stream")
+    ap.add_argument("--slots", type=int, default=8, help="serve: streams (requests in flight)")
+    ap.add_argument("--budget-gib", type=float, default=0.0, help="serve: HBM budget (0: 85%% of free memory)")
+    ap.add_argument("--max-dim", type=int, default=16384, help="serve: largest m, n, k")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -222,6 +313,8 @@ def main():
     F.check(F.lib().fmmhip_init(), "init")
     if args.form == "batch":
         return run_batch(args, F)
+    if args.form == "serve":
+        return run_serve(args, F)
     n = args.size
     m = k = n
     if args.alg == "classical":

@@ -213,7 +213,6 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % WM) * (FI * 16), wn = (wave / WM) * (FJ * 16);
-    const int S = a.R * a.nkb;
 
     // LDS chunk of the thread's staging row/chunk (the row swizzle depends on tid only)
     const int st_base = (tid >> 2) * 4 + ((tid & 3) ^ ((tid >> 3) & 3));   // lds_chunk(tid >> 2, tid & 3)

@@ -256,7 +256,7 @@ constexpr double kWsFractionOfFree = 0.5;
 
 int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
-                 hipStream_t st, Prepared* P)
+                 hipStream_t st, Prepared* P, DescSource* src)
 {
     memset(P, 0, sizeof(*P));
     P->kind = kind;
@@ -304,8 +304,24 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         presum = hipMemGetInfo(&fr, &tot) == hipSuccess && ws_bytes < kWsFractionOfFree * (double)fr;
     }
 
-    DevDesc* d = nullptr;
-    int rc = get_desc(alg, g, presum, a_mn, b_mn, st, &d);
+    DescView dv;
+    memset(&dv, 0, sizeof(dv));
+    int rc;
+    if (src) {
+        rc = src->get(alg, g, presum, a_mn, b_mn, st, &dv);
+    } else {
+        DevDesc* d = nullptr;
+        rc = get_desc(alg, g, presum, a_mn, b_mn, st, &d);
+        if (!rc) {
+            dv.fused_ptr = d->fused.dptr();
+            dv.fused_ent = d->fused.dent();
+            dv.fused_ent_h = d->fused.ent_h;
+            if (presum) {
+                dv.presum_ptr = d->presum.dptr();
+                dv.presum_ent = d->presum.dent();
+            }
+        }
+    }
     if (rc) return rc;
 
     KArgs& ka = P->ka;
@@ -325,19 +341,19 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         const int R = alg->R;
         fmmhip_presum_seg_t& sa = P->segA;
         sa.X = A; sa.S = (void*)(uintptr_t)0; sa.ld = a_mn ? csA : rsA;
-        sa.ptr = d->fused.dptr(); sa.ent = d->fused.dent();
+        sa.ptr = dv.fused_ptr; sa.ent = dv.fused_ent;
         sa.R = R; sa.swap = a_mn ? 0 : 1; sa.Lc = wa.Lc; sa.Ls = wa.Ls; sa.ldo = wa.ldo;
         fmmhip_presum_seg_t& sb = P->segB;
         sb.X = B; sb.S = (void*)(uintptr_t)offB; sb.ld = b_mn ? rsB : csB;
-        sb.ptr = d->fused.dptr() + (R + 1); sb.ent = d->fused.dent();
+        sb.ptr = dv.fused_ptr + (R + 1); sb.ent = dv.fused_ent;
         sb.R = R; sb.swap = b_mn ? 1 : 0; sb.Lc = wb.Lc; sb.Ls = wb.Ls; sb.ldo = wb.ldo;
         ka.A = (const void*)(uintptr_t)0;
         ka.B = (const void*)(uintptr_t)offB;
         ka.lda = wa.ldo;
         ka.ldb = wb.ldo;
         ka.min_m = (int)g.lm; ka.min_k = (int)g.lk; ka.min_n = (int)g.ln;   // full leaves
-        ka.ptr = d->presum.dptr();
-        ka.ent = d->presum.dent();
+        ka.ptr = dv.presum_ptr;
+        ka.ent = dv.presum_ent;
         ka.va = ka.vb = 1;           // workspace leaves are 16-byte aligned with even ldo
         if (shape < 0) shape = 0;    // one part per product
     } else {
@@ -346,10 +362,10 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.lda = a_mn ? csA : rsA;
         ka.ldb = b_mn ? rsB : csB;
         ka.min_m = (int)(m / alg->mt); ka.min_k = (int)(k / alg->kt); ka.min_n = (int)(n / alg->nt);
-        ka.ptr = d->fused.dptr();
-        ka.ent = d->fused.dent();
-        ka.va = pairs_aligned(A, esz, ka.lda, d->fused.ent_h, 0, alg->nnz_u);
-        ka.vb = pairs_aligned(B, esz, ka.ldb, d->fused.ent_h, alg->nnz_u, alg->nnz_u + alg->nnz_v);
+        ka.ptr = dv.fused_ptr;
+        ka.ent = dv.fused_ent;
+        ka.va = pairs_aligned(A, esz, ka.lda, dv.fused_ent_h, 0, alg->nnz_u);
+        ka.vb = pairs_aligned(B, esz, ka.ldb, dv.fused_ent_h, alg->nnz_u, alg->nnz_u + alg->nnz_v);
         if (shape < 0) shape = multi ? 1 : 0;
     }
     P->shape = shape;
@@ -458,6 +474,62 @@ int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int*
 }
 
 void* workspace_for(hipStream_t st, size_t bytes) { return workspace_get(st, bytes); }
+
+namespace {
+size_t list_bytes(int R, long nent)
+{
+    return ((sizeof(int32_t) * 3 * (size_t)(R + 1) + 255) & ~(size_t)255) + sizeof(fmm_entry_t) * (size_t)(nent + 1);
+}
+
+// Copies a host list into `host` at byte offset `off` in the device layout
+// of upload_list (CSR, then entries at the next 256-byte boundary).
+size_t pack_list(const DevList& l, int R, char* host, size_t off)
+{
+    const size_t ptr_bytes = sizeof(int32_t) * 3 * (size_t)(R + 1);
+    const size_t ent_off = (ptr_bytes + 255) & ~(size_t)255;
+    memcpy(host + off, l.ptr_h, ptr_bytes);
+    memcpy(host + off + ent_off, l.ent_h, sizeof(fmm_entry_t) * (size_t)l.nent);
+    return ent_off;
+}
+}  // namespace
+
+size_t desc_bound(fmmhip_alg_t alg)
+{
+    const long fused = alg->nnz_u + alg->nnz_v + alg->nnz_w;
+    const long pres = 2L * alg->R + alg->nnz_w;
+    return ((list_bytes(alg->R, fused) + 255) & ~(size_t)255) + list_bytes(alg->R, pres) + 256;
+}
+
+int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, void* host, const void* dev,
+              size_t cap, size_t* used, DescView* v)
+{
+    DevList fused, pres;
+    int rc = fmm_build_entries(alg, &g, &fused.ptr_h, &fused.ent_h, &fused.nent);
+    if (!rc && presum) rc = build_presum_list(alg, g, fused, a_mn, b_mn, pres);
+    size_t need = ((list_bytes(alg->R, fused.nent) + 255) & ~(size_t)255) + (presum ? list_bytes(alg->R, pres.nent) : 0);
+    if (!rc && need > cap) rc = FMMHIP_ERR_INVALID;
+    if (!rc) {
+        char* h = (char*)host;
+        const char* d = (const char*)dev;
+        const size_t e1 = pack_list(fused, alg->R, h, 0);
+        const size_t off2 = (list_bytes(alg->R, fused.nent) + 255) & ~(size_t)255;
+        v->fused_ptr = (const int*)d;
+        v->fused_ent = (const fmm_entry_t*)(d + e1);
+        v->fused_ent_h = (const fmm_entry_t*)(h + e1);
+        if (presum) {
+            const size_t e2 = pack_list(pres, alg->R, h, off2);
+            v->presum_ptr = (const int*)(d + off2);
+            v->presum_ent = (const fmm_entry_t*)(d + off2 + e2);
+        } else {
+            v->presum_ptr = nullptr;
+            v->presum_ent = nullptr;
+        }
+        *used = need;
+    }
+    free(fused.ptr_h); free(fused.ent_h);
+    free(pres.ptr_h); free(pres.ent_h);
+    return rc;
+}
 
 bool timing_active()
 {

@@ -26,10 +26,36 @@ struct Prepared {
     fmmhip_presum_seg_t segA, segB;
 };
 
+// Device coefficient lists of one call: the fused list (every nonzero part
+// of every product: CSR over U|V|W, then the entries), a host copy of its
+// entries, and in the pre-summed mode the one-part-per-product list.
+struct DescView {
+    const int* fused_ptr;
+    const fmm_entry_t* fused_ent;
+    const fmm_entry_t* fused_ent_h;
+    const int* presum_ptr;
+    const fmm_entry_t* presum_ent;
+};
+
+// Where prepare_call takes the lists from; without one, from the
+// algorithm's per-geometry cache.
+struct DescSource {
+    virtual int get(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, hipStream_t st,
+                    DescView* v) = 0;
+    virtual ~DescSource() {}
+};
+
+// Upper bound of the device bytes of one call's lists, and the packing of
+// them into `host` in their device layout with pointers bound to `dev`
+// (the caller copies host -> dev before the launch).
+size_t desc_bound(fmmhip_alg_t alg);
+int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, void* host, const void* dev,
+              size_t cap, size_t* used, DescView* v);
+
 // Resolves one call (pack: 0 fused, 1 pre-summed, -1 automatic).
 int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
-                 hipStream_t st, Prepared* P);
+                 hipStream_t st, Prepared* P, DescSource* src = nullptr);
 // Rebases a pre-summed call's workspace offsets onto `ws`.
 void bind_workspace(Prepared* P, void* ws);
 // Launches a prepared, bound call on its stream.

@@ -62,6 +62,22 @@ ObjT._fields_ = [("root", ctypes.POINTER(ObjT)), ("off", ctypes.c_int64 * 2), ("
                  ("pd", ctypes.c_int64), ("m_panel", ctypes.c_int64), ("n_panel", ctypes.c_int64)]
 
 
+class Request(ctypes.Structure):
+    """fmmhip_request_t."""
+    _fields_ = [("is_double", ctypes.c_int), ("m", ctypes.c_long), ("n", ctypes.c_long), ("k", ctypes.c_long),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("A", ctypes.c_void_p), ("lda", ctypes.c_long),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_long), ("C", ctypes.c_void_p), ("ldc", ctypes.c_long),
+                ("cand", ctypes.c_int), ("keep", ctypes.c_int), ("arrival_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    """fmmhip_server_stats_t."""
+    _fields_ = [("submitted", ctypes.c_long), ("completed", ctypes.c_long), ("failed", ctypes.c_long),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("span_s", ctypes.c_double), ("arena_bytes", ctypes.c_size_t),
+                ("arena_peak", ctypes.c_size_t), ("max_inflight", ctypes.c_int)]
+
+
 class FmmError(RuntimeError):
     pass
 
@@ -123,6 +139,16 @@ _SIGS = {
     "fmmhip_tuner_calibrate": (_I, [_P]),
     "fmmhip_tuner_save": (_I, [ctypes.c_char_p]),
     "fmmhip_tuner_load": (_I, [ctypes.c_char_p]),
+    "fmmhip_server_create": (_I, [ctypes.POINTER(_P), ctypes.c_size_t, _I]),
+    "fmmhip_server_now": (_D, [_P]),
+    "fmmhip_server_submit": (_I, [_P, ctypes.POINTER(Request), ctypes.POINTER(_L)]),
+    "fmmhip_server_wait": (_I, [_P, _L, ctypes.POINTER(_I)]),
+    "fmmhip_server_result": (_I, [_P, _L, ctypes.POINTER(_P), ctypes.POINTER(_L), ctypes.POINTER(_D),
+                                  ctypes.POINTER(_I)]),
+    "fmmhip_server_release": (_I, [_P, _L]),
+    "fmmhip_server_drain": (_I, [_P]),
+    "fmmhip_server_stats": (_I, [_P, ctypes.POINTER(ServerStats)]),
+    "fmmhip_server_destroy": (None, [_P]),
     "fmmhip_malloc": (_I, [ctypes.POINTER(_P), ctypes.c_size_t]),
     "fmmhip_free": (_I, [_P]),
     "fmmhip_memcpy_h2d": (_I, [_P, _P, ctypes.c_size_t]),
@@ -364,6 +390,79 @@ class Batch:
     def close(self):
         if self.h:
             lib().fmmhip_batch_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving form (fmmhip_server_*): requests over device-resident
+    DeviceMatrix operands, admitted by HBM budget onto `slots` streams."""
+
+    def __init__(self, hbm_budget, slots=4):
+        self.h = _P()
+        check(lib().fmmhip_server_create(ctypes.byref(self.h), hbm_budget, slots), "server_create")
+
+    def now(self):
+        return lib().fmmhip_server_now(self.h)
+
+    def submit(self, m, n, k, alpha, A, B, beta=0.0, C=None, cand=-1, keep=1, arrival_s=0.0, lda=None, ldb=None):
+        """A and B are DeviceMatrix objects or (pointer, ld) pairs; C a
+        DeviceMatrix or None (server-allocated, beta 0).  Returns the id."""
+        def ptr_ld(X, ld):
+            if isinstance(X, tuple):
+                return X
+            return X.ptr.value if hasattr(X.ptr, "value") else X.ptr, X.ld
+        pa, la = ptr_ld(A, lda)
+        pb, lb = ptr_ld(B, ldb)
+        dt = A.dtype if not isinstance(A, tuple) else None
+        r = Request()
+        r.is_double = 1 if (dt is None or dt.itemsize == 8) else 0
+        r.m, r.n, r.k, r.alpha, r.beta = m, n, k, alpha, beta
+        r.A, r.lda, r.B, r.ldb = pa, la, pb, lb
+        if C is not None:
+            pc = C.ptr.value if hasattr(C.ptr, "value") else C.ptr
+            r.C, r.ldc = pc, C.ld
+        r.cand, r.keep, r.arrival_s = cand, keep, arrival_s
+        rid = _L()
+        check(lib().fmmhip_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)), "server_submit")
+        return rid.value
+
+    def submit_request(self, req):
+        rid = _L()
+        check(lib().fmmhip_server_submit(self.h, ctypes.byref(req), ctypes.byref(rid)), "server_submit")
+        return rid.value
+
+    def wait(self, rid):
+        st = _I()
+        check(lib().fmmhip_server_wait(self.h, rid, ctypes.byref(st)), "server_wait")
+        return st.value
+
+    def result(self, rid):
+        """(status, C pointer, ldc, latency ms, candidate)."""
+        c, ld, lat, cand = _P(), _L(), _D(), _I()
+        st = lib().fmmhip_server_result(self.h, rid, ctypes.byref(c), ctypes.byref(ld), ctypes.byref(lat),
+                                        ctypes.byref(cand))
+        return st, c.value, ld.value, lat.value, cand.value
+
+    def release(self, rid):
+        check(lib().fmmhip_server_release(self.h, rid), "server_release")
+
+    def drain(self):
+        check(lib().fmmhip_server_drain(self.h), "server_drain")
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().fmmhip_server_stats(self.h, ctypes.byref(s)), "server_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().fmmhip_server_destroy(self.h)
             self.h = _P()
 
     def __del__(self):

@@ -294,6 +294,52 @@ int fmmhip_tuner_calibrate(void* stream);
 int fmmhip_tuner_save(const char* path);
 int fmmhip_tuner_load(const char* path);
 
+/* ---- serving form: a request loop with continuous batching ---- */
+
+/* One GEMM request, C := beta*C + alpha*A*B, column-major, operands resident
+ * on the device.  C == NULL: the server allocates C (ldc = m) in its arena
+ * (beta must be 0); keep = 1 keeps it until fmmhip_server_release, keep = 0
+ * drops it at completion (latency/throughput only).  cand: -1 tuner, 0
+ * classical, 1/2 levels of Strassen.  arrival_s: the time, on the server's
+ * clock (fmmhip_server_now), at which the request arrives; earlier means now. */
+typedef struct {
+    int is_double;
+    long m, n, k;
+    double alpha, beta;
+    const void* A; long lda;
+    const void* B; long ldb;
+    void* C; long ldc;
+    int cand;
+    int keep;
+    double arrival_s;
+} fmmhip_request_t;
+
+typedef struct {
+    long submitted, completed, failed;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* arrival -> observed completion */
+    double span_s;                            /* first arrival -> last completion */
+    size_t arena_bytes, arena_peak;
+    int max_inflight;
+} fmmhip_server_stats_t;
+
+typedef struct fmmhip_server_s* fmmhip_server_t;
+/* A server with an HBM budget of `hbm_budget` bytes (one arena, allocated
+ * here) and `slots` streams (requests in flight at once). */
+int    fmmhip_server_create(fmmhip_server_t* out, size_t hbm_budget, int slots);
+double fmmhip_server_now(fmmhip_server_t s);
+/* Queues a request (thread-safe, returns at once). */
+int    fmmhip_server_submit(fmmhip_server_t s, const fmmhip_request_t* req, long* id);
+/* Blocks until request `id` is done; *status = 0 or its FMMHIP_ERR_* code. */
+int    fmmhip_server_wait(fmmhip_server_t s, long id, int* status);
+/* A done request's C, ldc, latency and candidate; returns its status. */
+int    fmmhip_server_result(fmmhip_server_t s, long id, void** C, long* ldc, double* latency_ms, int* cand);
+/* Forgets a done request (frees a kept server-allocated C). */
+int    fmmhip_server_release(fmmhip_server_t s, long id);
+/* Blocks until every submitted request is done. */
+int    fmmhip_server_drain(fmmhip_server_t s);
+int    fmmhip_server_stats(fmmhip_server_t s, fmmhip_server_stats_t* st);
+void   fmmhip_server_destroy(fmmhip_server_t s);
+
 /* Device memory, streams, events (thin wrappers so that C and ctypes
  * callers need no HIP headers). */
 int fmmhip_malloc(void** p, size_t bytes);

@@ -246,3 +246,9 @@ def test_tuner_table_save_load_round_trip(tmp_path):
 def test_batch_create_without_device_fails():
     h = ctypes.c_void_p()
     assert F.lib().fmmhip_batch_create(ctypes.byref(h)) == 5       # FMMHIP_ERR_NODEVICE
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the behaviour without a device")
+def test_server_create_without_device_fails():
+    h = ctypes.c_void_p()
+    assert F.lib().fmmhip_server_create(ctypes.byref(h), 1 << 30, 4) == 5       # FMMHIP_ERR_NODEVICE
