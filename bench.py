@@ -194,6 +194,61 @@ def run_batch(args, F):
     tuned.close()
 
 
+def run_sweep(args, F):
+    """FP64 C += A*B with every algorithm at 1 and 2 levels (automatic pack
+    mode) and the classical kernel on one shape: effective GFLOPS and the
+    normalised error against the classical product of the same operands."""
+    m = n = args.size
+    k = args.k or args.size
+    A = F.DeviceMatrix(m, k).fill_uniform(0, 0)
+    B = F.DeviceMatrix(k, n).fill_uniform(0, 1)
+    C = F.DeviceMatrix(m, n)
+    Cr = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+    F.dgemm(m, n, k, 1.0, A, B, 1.0, Cr)
+    F.sync()
+    nA, nB = F.fro(A), F.fro(B)
+    flops = 2.0 * m * n * k
+
+    def timed(call, steps):
+        call()
+        F.sync()
+        evs = [(F.Event(), F.Event()) for _ in range(steps)]
+        for e0, e1 in evs:
+            e0.record()
+            call()
+            e1.record()
+        F.sync()
+        return sum(e0.elapsed_ms(e1) for e0, e1 in evs) / steps
+
+    cls_ms = timed(lambda: F.dgemm(m, n, k, 1.0, A, B, 1.0, C), args.steps)
+    rows = [{"alg": "classical", "levels": 0, "ms": round(cls_ms, 3), "gflops": round(flops / cls_ms / 1e6, 1),
+             "rel_err": 0.0, "vs_classical": 1.0}]
+    for name in ["222", "232", "322", "333", "444"]:
+        f = F.strassen() if name == "222" else \
+            F.new_fmm(os.path.join(ROOT, "tests", "golden", "algorithms", f"{name}.txt"))
+        for levels in (1, 2):
+            alg = F.Alg([f] * levels)
+            ms = timed(lambda: F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, C), args.steps)
+            C.fill_uniform(0, 2)
+            F.dgemm_fmm(alg, m, n, k, 1.0, A, B, 1.0, C)
+            F.sync()
+            err = F.fro_diff(C, Cr) / (nA * nB)
+            rows.append({"alg": name, "levels": levels, "ms": round(ms, 3), "gflops": round(flops / ms / 1e6, 1),
+                         "rel_err": err, "vs_classical": round(cls_ms / ms, 4)})
+            alg.close()
+            F.check(F.lib().fmmhip_workspace_release(), "workspace_release")
+    best = max(rows, key=lambda r: r["gflops"])
+    out = {
+        "metric": "effective FP64 GFLOPS (2mnk/t), FMM algorithm sweep",
+        "value": best["gflops"], "unit": "GFLOPS", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (SplitMix64 uniform[-1,1], seed 0, device-resident)",
+        "config": {"workload": f"FP64 C += A*B, m={m} n={n} k={k}, column-major, every algorithm at 1 and 2 levels",
+                   "m": m, "n": n, "k": k, "best": f"{best['alg']} L{best['levels']}", "results": rows},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def serve_stream(count, rate, seed, max_dim):
     """(dtype, m, n, k, arrival offset s): m, n, k log-uniform in
     [128, max_dim] rounded to multiples of 32, 70% FP64, Poisson arrivals."""
@@ -293,9 +348,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against the classical kernel")
     ap.add_argument("--profile", action="store_true")
-    ap.add_argument("--form", default="single", choices=["single", "batch", "serve"],
+    ap.add_argument("--form", default="single", choices=["single", "batch", "serve", "sweep"],
                     help="single: one m=n=k product; batch: 512 uneven FP64/FP32 GEMMs in one graph; "
-                         "serve: Poisson request stream through the continuous-batching server")
+                         "serve: Poisson request stream through the continuous-batching server; "
+                         "sweep: every algorithm at 1 and 2 levels on one FP64 shape")
+    ap.add_argument("--k", type=int, default=0, help="sweep/single: k if not --size (rank-k shapes)")
+    ap.add_argument("--no-bf16x3", action="store_true", help="single: skip the BF16x3 line beside the FP64 one")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--rate", type=float, default=500.0, help="serve: arrivals per second")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the timed stream")
@@ -315,6 +373,8 @@ def main():
         return run_batch(args, F)
     if args.form == "serve":
         return run_serve(args, F)
+    if args.form == "sweep":
+        return run_sweep(args, F)
     n = args.size
     m = k = n
     if args.alg == "classical":
@@ -395,6 +455,29 @@ def main():
         F.dgemm(m, n, k, 1.0, A, B, 1.0, Cr)
         F.sync()
         rel = F.fro_diff(Cf, Cr) / (F.fro(A) * F.fro(B))
+        Cf.free()
+
+    # the precision form beside the FP64 line: BF16x3 (eight plane products)
+    # with the classical leaf on the same operands, its error against the
+    # classical FP64 product
+    bf3 = None
+    if args.leaf == "f64" and not args.no_bf16x3 and not args.no_check:
+        cls_alg = F.Alg(classical=True)
+        Cb = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+        F.dgemm_fmm(cls_alg, m, n, k, 1.0, A, B, 1.0, Cb, leaf=F.LEAF_BF16X3)
+        F.sync()
+        b_err = F.fro_diff(Cb, Cr) / (F.fro(A) * F.fro(B))
+        be = [(F.Event(), F.Event()) for _ in range(3)]
+        for e0, e1 in be:
+            e0.record()
+            F.dgemm_fmm(cls_alg, m, n, k, 1.0, A, B, 1.0, Cb, leaf=F.LEAF_BF16X3)
+            e1.record()
+        F.sync()
+        b_ms = sum(e0.elapsed_ms(e1) for e0, e1 in be) / len(be)
+        bf3 = {"leaf": "bf16x3 (8 plane products), classical", "ms": round(b_ms, 3),
+               "effective_gflops": round(2.0 * m * n * k / (b_ms * 1e-3) / 1e9, 1), "rel_err_vs_classical": b_err}
+        Cb.free()
+        F.check(F.lib().fmmhip_workspace_release(), "workspace_release")
 
     fl = leaf_flops(info, m, n, k)
     peak = {"f64": FP64_PEAK_TFLOPS, "f32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS,
@@ -425,6 +508,7 @@ def main():
             "fmm_over_classical": round(eff / cls_gflops, 4) if cls_gflops else None,
             "rel_err_vs_classical": rel,
             "summation_ms_per_step": round(presum_ms, 3),
+            "precision_form": bf3,
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3),
                        "mean": round(sum(lat) / len(lat), 3)},
