@@ -249,6 +249,67 @@ def run_sweep(args, F):
     print(json.dumps(out), flush=True)
 
 
+def run_precision(args, F):
+    """The precision form: classical, 1- and 2-level Strassen with FP64,
+    FP32, BF16x3 (eight plane products) and BF16x3 (six) leaves at 8192^3
+    and 16384^3, on uniform[-1,1] inputs and on the stress variant (rows of
+    A and columns of B scaled by 10^U(-4,4)); effective GFLOPS and the
+    normalised error against the classical FP64 product of the same
+    operands."""
+    import numpy as np
+    leaves = [("f64", F.LEAF_NATIVE), ("f32", F.LEAF_F32), ("bf16x3", F.LEAF_BF16X3), ("bf16x3_6", F.LEAF_BF16X3_6)]
+    rows = []
+    for n in (8192, 16384):
+        for variant in ("uniform", "stress"):
+            A = F.DeviceMatrix(n, n).fill_uniform(0, 0)
+            B = F.DeviceMatrix(n, n).fill_uniform(0, 1)
+            if variant == "stress":
+                rng = np.random.default_rng(5)
+                rs, cs = F.DeviceBuffer(8 * n), F.DeviceBuffer(8 * n)
+                rs.upload(10.0 ** rng.uniform(-4, 4, n))
+                cs.upload(10.0 ** rng.uniform(-4, 4, n))
+                F.check(F.lib().fmmhip_scale_rows_cols(A.ptr, n, n, A.ld, rs.ptr, None, None), "scale")
+                F.check(F.lib().fmmhip_scale_rows_cols(B.ptr, n, n, B.ld, None, cs.ptr, None), "scale")
+            Cr = F.DeviceMatrix(n, n)
+            F.dgemm(n, n, n, 1.0, A, B, 0.0, Cr)
+            C = F.DeviceMatrix(n, n)
+            F.sync()
+            nA, nB = F.fro(A), F.fro(B)
+            for name, levels in (("classical", 0), ("222", 1), ("222", 2)):
+                alg = F.Alg(classical=True) if levels == 0 else F.Alg([F.strassen()] * levels)
+                for lname, leaf in leaves:
+                    F.dgemm_fmm(alg, n, n, n, 1.0, A, B, 0.0, C, leaf=leaf)
+                    F.sync()
+                    err = F.fro_diff(C, Cr) / (nA * nB)
+                    evs = [(F.Event(), F.Event()) for _ in range(2)]
+                    for e0, e1 in evs:
+                        e0.record()
+                        F.dgemm_fmm(alg, n, n, n, 1.0, A, B, 0.0, C, leaf=leaf)
+                        e1.record()
+                    F.sync()
+                    ms = sum(e0.elapsed_ms(e1) for e0, e1 in evs) / len(evs)
+                    rows.append({"n": n, "inputs": variant, "alg": name, "levels": levels, "leaf": lname,
+                                 "ms": round(ms, 3), "effective_gflops": round(2.0 * n ** 3 / ms / 1e6, 1),
+                                 "rel_err": err})
+                alg.close()
+                F.check(F.lib().fmmhip_workspace_release(), "workspace_release")
+            for X in (A, B, C, Cr):
+                X.free()
+    ok = [r for r in rows if r["n"] == 16384 and r["inputs"] == "uniform" and r["rel_err"] <= 1e-10]
+    best = max(ok, key=lambda r: r["effective_gflops"])
+    out = {
+        "metric": "precision form: effective GFLOPS (2mnk/t) at normalised error <= 1e-10, 16384^3",
+        "value": best["effective_gflops"], "unit": "GFLOPS", "n_gpus": 1, "steps": 2, "warmup": 1,
+        "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 operands; f64/f32/bf16 leaves",
+        "data": "synthetic (SplitMix64 uniform[-1,1], seed 0; stress: rows of A, columns of B x 10^U(-4,4), seed 5)",
+        "config": {"workload": "C = A*B, m=n=k in {8192, 16384}, classical / 222 L1 / 222 L2 x leaves f64, f32, "
+                               "bf16x3 (8 products), bf16x3_6 (6 products)",
+                   "best": f"{best['alg']} L{best['levels']} {best['leaf']}", "results": rows},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def serve_stream(count, rate, seed, max_dim):
     """(dtype, m, n, k, arrival offset s): m, n, k log-uniform in
     [128, max_dim] rounded to multiples of 32, 70% FP64, Poisson arrivals."""
@@ -348,10 +409,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against the classical kernel")
     ap.add_argument("--profile", action="store_true")
-    ap.add_argument("--form", default="single", choices=["single", "batch", "serve", "sweep"],
+    ap.add_argument("--form", default="single", choices=["single", "batch", "serve", "sweep", "precision"],
                     help="single: one m=n=k product; batch: 512 uneven FP64/FP32 GEMMs in one graph; "
                          "serve: Poisson request stream through the continuous-batching server; "
-                         "sweep: every algorithm at 1 and 2 levels on one FP64 shape")
+                         "sweep: every algorithm at 1 and 2 levels on one FP64 shape; "
+                         "precision: FP64/FP32/BF16x3 leaves at 8192^3 and 16384^3, uniform and scaled inputs")
     ap.add_argument("--k", type=int, default=0, help="sweep/single: k if not --size (rank-k shapes)")
     ap.add_argument("--no-bf16x3", action="store_true", help="single: skip the BF16x3 line beside the FP64 one")
     ap.add_argument("--batch-count", type=int, default=512)
@@ -375,6 +437,8 @@ def main():
         return run_serve(args, F)
     if args.form == "sweep":
         return run_sweep(args, F)
+    if args.form == "precision":
+        return run_precision(args, F)
     n = args.size
     m = k = n
     if args.alg == "classical":
@@ -403,7 +467,7 @@ def main():
         step()
         gib = 1 << 30
         src, dst = F.DeviceBuffer(gib), F.DeviceBuffer(gib)
-        F.check(F.lib().fmmhip_memcpy_d2d(dst.ptr, src.ptr, gib, None), "copy")
+        F.check(F.lib().fmmhip_copy_kernel(dst.ptr, src.ptr, gib, None), "copy")
         F.sync()
         print(json.dumps({"profile": tag, "steps": 2, "calibration_copy_bytes": gib}), flush=True)
         return

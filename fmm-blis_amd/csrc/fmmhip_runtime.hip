@@ -5,6 +5,7 @@
 // and thread set-up (blis/frame/base/bli_init.c, bli_obj.c).  Errors are
 // returned as fmmhip_status_t codes, never by aborting.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <mutex>
 #include <string.h>
 #include <stdio.h>
@@ -118,6 +119,25 @@ extern "C" int fmmhip_memcpy_d2d(void* d, const void* s, size_t b, void* stream)
     HIP_OK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, S(stream)));
     return 0;
 }
+// A plain streaming copy kernel (16 bytes per lane per iteration): the
+// reference workload of known size for calibrating HBM byte counters.
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+extern "C" int fmmhip_copy_kernel(void* d, const void* s, size_t b, void* stream)
+{
+    INIT_OR_RETURN();
+    if (!d || !s || b % 16 || ((uintptr_t)d | (uintptr_t)s) % 16) return FMMHIP_ERR_INVALID;
+    const long n = (long)(b / 16);
+    if (!n) return 0;
+    const long blocks = std::min<long>((n + 255) / 256, 256L * 8 * 4);
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, S(stream), (const uint4*)s, (uint4*)d, n);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int fmmhip_memset(void* p, int v, size_t b)
 {
     INIT_OR_RETURN();

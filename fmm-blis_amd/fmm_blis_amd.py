@@ -121,6 +121,7 @@ _SIGS = {
     "fmmhip_timing_enable": (_I, [_I]),
     "fmmhip_timing_read": (_I, [ctypes.POINTER(_L)] + [ctypes.POINTER(_D)] * 3),
     "fmmhip_memcpy_d2d": (_I, [_P, _P, ctypes.c_size_t, _P]),
+    "fmmhip_copy_kernel": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),

@@ -349,6 +349,10 @@ int fmmhip_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int fmmhip_memset(void* p, int value, size_t bytes);
 /* Device-to-device copy, asynchronous on `stream`. */
 int fmmhip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* Device-to-device copy by a plain streaming kernel (bytes and both
+ * pointers multiples of 16): a workload of known size for calibrating the
+ * HBM byte counters of a profiler. */
+int fmmhip_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 int fmmhip_stream_create(void** stream);
 int fmmhip_stream_destroy(void* stream);
 int fmmhip_stream_sync(void* stream);
