@@ -40,16 +40,20 @@ typedef float f2_t __attribute__((ext_vector_type(2)));
 constexpr int BM = 128;          // tile rows (leaf M)
 constexpr int BN = 128;          // tile cols (leaf N)
 constexpr int BK = 16;           // k per LDS stage
-// LDS tiles are kept [BK][LDS_ROW] (mn innermost): rows of 16 lanes read 128
-// contiguous bytes and the +16 pad puts consecutive k rows in opposite bank
-// halves.  (A k-major layout for k-contiguous operands removes the 8-way
-// conflicts of their transposing store but costs two registers, which
-// spills at two workgroups per CU; measured no faster.)
-constexpr int LDS_ROW = BM + 16;
-constexpr int LDS_TILE = BK * LDS_ROW;   // elements per tile buffer
+// LDS tiles are kept [BK][row] (mn innermost): rows of 16 lanes read 128
+// contiguous bytes.  The transposing store of a k-contiguous operand writes
+// 8 even k rows of one mn column per 8 lanes; the row stride spreads them
+// over the banks: BM + 1 doubles (8 distinct bank pairs per 8 lanes) and
+// BM + 2 floats (32 distinct banks per 32 lanes).  A stride of BM + 16 put
+// all 8 rows in one bank (8-way conflicts, 6% of the FP64 kernel's time).
+template <typename TL>
+__device__ __host__ constexpr int lds_row() { return sizeof(TL) == 8 ? BM + 1 : BM + 2; }
+template <typename TL>
+__device__ __host__ constexpr int lds_tile() { return BK * lds_row<TL>(); }   // elements per tile buffer
 
 // Element (mn, k) of an LDS tile.
-__device__ __forceinline__ constexpr int lds_at(int mn, int k) { return k * LDS_ROW + mn; }
+template <typename TL>
+__device__ __forceinline__ constexpr int lds_at(int mn, int k) { return k * lds_row<TL>() + mn; }
 
 // Launch shape: NT threads as a WAVES_M x WAVES_N grid of waves, OCC waves
 // per SIMD.
@@ -244,12 +248,12 @@ struct Stage {
     template <typename TL>
     __device__ __forceinline__ void store(TL* lds) const
     {
-        TL* q = lds + lds_at(lmn0, lk0);
+        TL* q = lds + lds_at<TL>(lmn0, lk0);
 #pragma unroll
         for (int s = 0; s < PAIRS; s++) {
-            TL* qs = q + lds_at(dmn(s), dk(s));
+            TL* qs = q + lds_at<TL>(dmn(s), dk(s));
             qs[0] = (TL)sum[s][0];
-            qs[lds_at(MN_CONTIG ? 1 : 0, MN_CONTIG ? 0 : 1)] = (TL)sum[s][1];   // next along the contiguous dim
+            qs[lds_at<TL>(MN_CONTIG ? 1 : 0, MN_CONTIG ? 0 : 1)] = (TL)sum[s][1];   // next along the contiguous dim
         }
     }
 };
@@ -278,7 +282,8 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
 // body of both the one-problem kernel and the batched persistent kernel.
 // sA/sB are the workgroup's double-buffered LDS tiles.
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
-__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*sA)[LDS_TILE], TL (*sB)[LDS_TILE])
+__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*sA)[lds_tile<TL>()],
+                                          TL (*sB)[lds_tile<TL>()])
 {
     constexpr int WAVES_M = Cfg<SHAPE>::WAVES_M;
     constexpr bool MULTI = Cfg<SHAPE>::MULTI;
@@ -374,9 +379,9 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*s
                 const int kk = 4 * ks + (lane >> 4);
                 TL fa[FI], fb[FJ];
 #pragma unroll
-                for (int i = 0; i < FI; i++) fa[i] = la[lds_at(wm + 16 * i + (lane & 15), kk)];
+                for (int i = 0; i < FI; i++) fa[i] = la[lds_at<TL>(wm + 16 * i + (lane & 15), kk)];
 #pragma unroll
-                for (int j = 0; j < FJ; j++) fb[j] = lb[lds_at(wn + 16 * j + (lane & 15), kk)];
+                for (int j = 0; j < FJ; j++) fb[j] = lb[lds_at<TL>(wn + 16 * j + (lane & 15), kk)];
 #pragma unroll
                 for (int i = 0; i < FI; i++)
 #pragma unroll
@@ -467,8 +472,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>
                           amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
 void fmm_gemm_kernel(KArgs a)
 {
-    __shared__ TL sA[2][LDS_TILE];
-    __shared__ TL sB[2][LDS_TILE];
+    __shared__ TL sA[2][lds_tile<TL>()];
+    __shared__ TL sB[2][lds_tile<TL>()];
     int tm, tn;
     tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
     gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL>(a, tm, tn, sA, sB);
@@ -489,8 +494,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>
 void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items, int nitems,
                       int* __restrict__ head)
 {
-    __shared__ TL sA[2][LDS_TILE];
-    __shared__ TL sB[2][LDS_TILE];
+    __shared__ TL sA[2][lds_tile<TL>()];
+    __shared__ TL sB[2][lds_tile<TL>()];
     __shared__ int slot;
     for (;;) {
         if (threadIdx.x == 0) slot = atomicAdd(head, 1);
