@@ -80,7 +80,8 @@ typedef struct fmmhip_presum_seg_s {
     const int* ptr;             /* device CSR over products (R+1) */
     const fmm_entry_t* ent;     /* device entries */
     int R, swap, Lc, Ls, ldo;
-    int pad_;
+    int np;                     /* parts of the operand (entries' `part` < np) */
+    const fmm_entry_t* parts;   /* device table of the np parts (offset and extents), or NULL */
 } fmmhip_presum_seg_t;
 
 /* Batched summation: segment s is processed by blocks [boff[s], boff[s+1]),
@@ -93,6 +94,9 @@ int fmmhip_presum_d(const double* X, long ld, const int* ptr, const fmm_entry_t*
                     int Lc, int Ls, int ldo, double* S, void* stream);
 int fmmhip_presum_s(const float* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,
                     int Lc, int Ls, int ldo, float* S, void* stream);
+/* The same sums from a part table: every part element is read once and all
+ * R sums are formed from it (identical arithmetic, fewer HBM reads). */
+int fmmhip_presum_seg(int is_double, const fmmhip_presum_seg_t* seg, void* stream);
 
 #ifdef __cplusplus
 }

@@ -111,7 +111,10 @@ int build_presum_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const
 {
     const int R = alg->R;
     const WsLayout wa = ws_layout_a(g, a_mn), wb = ws_layout_b(g, b_mn);
-    out.nent = 2L * R + alg->nnz_w;
+    // entries: R A leaves, R B leaves, the W entries, then the part tables
+    // of A (mt*kt) and B (kt*nt) for the part-staged summation
+    const int npa = alg->mt * alg->kt, npb = alg->kt * alg->nt;
+    out.nent = 2L * R + alg->nnz_w + npa + npb;
     out.ptr_h = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
     out.ent_h = (fmm_entry_t*)calloc((size_t)out.nent + 1, sizeof(fmm_entry_t));
     if (!out.ptr_h || !out.ent_h) return FMMHIP_ERR_NOMEM;
@@ -135,6 +138,11 @@ int build_presum_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const
         eb.part = -1;
     }
     memcpy(out.ent_h + 2 * R, fused.ent_h + alg->nnz_u + alg->nnz_v, sizeof(fmm_entry_t) * (size_t)alg->nnz_w);
+    // part tables (unreferenced parts keep zero extents: nothing is read)
+    fmm_entry_t* pa = out.ent_h + 2 * R + alg->nnz_w;
+    fmm_entry_t* pb = pa + npa;
+    for (long e = 0; e < alg->nnz_u; e++) pa[fused.ent_h[e].part] = fused.ent_h[e];
+    for (long e = alg->nnz_u; e < alg->nnz_u + alg->nnz_v; e++) pb[fused.ent_h[e].part] = fused.ent_h[e];
     return FMMHIP_SUCCESS;
 }
 
@@ -249,6 +257,9 @@ void tm_record(hipEvent_t* ev, hipStream_t st)
     else *ev = nullptr;
 }
 
+// Largest part count of the part-staged summation kernel.
+constexpr int kPartStagedMax = 256;
+
 // Largest workspace the automatic choice takes for the pre-summed mode.
 constexpr double kWsFractionOfFree = 0.5;
 
@@ -343,10 +354,14 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         sa.X = A; sa.S = (void*)(uintptr_t)0; sa.ld = a_mn ? csA : rsA;
         sa.ptr = dv.fused_ptr; sa.ent = dv.fused_ent;
         sa.R = R; sa.swap = a_mn ? 0 : 1; sa.Lc = wa.Lc; sa.Ls = wa.Ls; sa.ldo = wa.ldo;
+        sa.np = alg->mt * alg->kt;
+        sa.parts = dv.presum_ent + 2 * R + alg->nnz_w;
         fmmhip_presum_seg_t& sb = P->segB;
         sb.X = B; sb.S = (void*)(uintptr_t)offB; sb.ld = b_mn ? rsB : csB;
         sb.ptr = dv.fused_ptr + (R + 1); sb.ent = dv.fused_ent;
         sb.R = R; sb.swap = b_mn ? 1 : 0; sb.Lc = wb.Lc; sb.Ls = wb.Ls; sb.ldo = wb.ldo;
+        sb.np = alg->kt * alg->nt;
+        sb.parts = sa.parts + sa.np;
         ka.A = (const void*)(uintptr_t)0;
         ka.B = (const void*)(uintptr_t)offB;
         ka.lda = wa.ldo;
@@ -406,7 +421,12 @@ int launch_prepared(const Prepared* P)
     TimedCall tc{nullptr, nullptr, nullptr, P->presum};
     if (timed) tm_record(&tc.e0, st);
     int rc = FMMHIP_SUCCESS;
-    if (P->presum) {
+    if (P->presum && P->segA.parts && P->segA.np <= kPartStagedMax && P->segB.np <= kPartStagedMax) {
+        const bool dbl = P->kind != Kind::S32;
+        rc = fmmhip_presum_seg(dbl, &P->segA, st);
+        if (!rc) rc = fmmhip_presum_seg(dbl, &P->segB, st);
+        if (rc) return rc;
+    } else if (P->presum) {
         const fmmhip_presum_seg_t& a = P->segA;
         const fmmhip_presum_seg_t& b = P->segB;
         if (P->kind == Kind::S32) {
@@ -496,7 +516,7 @@ size_t pack_list(const DevList& l, int R, char* host, size_t off)
 size_t desc_bound(fmmhip_alg_t alg)
 {
     const long fused = alg->nnz_u + alg->nnz_v + alg->nnz_w;
-    const long pres = 2L * alg->R + alg->nnz_w;
+    const long pres = 2L * alg->R + alg->nnz_w + alg->mt * alg->kt + alg->kt * alg->nt;
     return ((list_bytes(alg->R, fused) + 255) & ~(size_t)255) + list_bytes(alg->R, pres) + 256;
 }
 

@@ -91,7 +91,76 @@ __global__ __launch_bounds__(256) void presum_batch_kernel(const fmmhip_presum_s
     }
 }
 
+// Part-staged form: block = TB consecutive elements along c of one s;
+// thread t loads element (c, s) of every part once into its own LDS slot
+// xs[p][t] (read back only by itself: no barrier), then forms every S_r with
+// the entries' coefficients in CSR order -- the same FMA sequence as
+// presum_kernel, so the sums are bit-identical; HBM reads drop from one per
+// (entry, element) to one per (part, element).
+template <typename T, int NPMAX, int TB>
+__global__ __launch_bounds__(TB) void presum_parts_kernel(const T* __restrict__ X, long ld,
+                                                          const int* __restrict__ ptr,
+                                                          const fmm_entry_t* __restrict__ ent,
+                                                          const fmm_entry_t* __restrict__ parts, int np, int R,
+                                                          int swap, int Lc, int Ls, int ldo, T* __restrict__ S)
+{
+    __shared__ double xs[NPMAX][TB];
+    const int t = threadIdx.x;
+    const long nbc = (Lc + TB - 1) / TB;
+    const long plane = (long)ldo * Ls;
+    for (long blk = blockIdx.x; blk < nbc * Ls; blk += gridDim.x) {
+        const int s = (int)(blk / nbc);
+        const int c = (int)(blk - (long)s * nbc) * TB + t;
+        for (int p = 0; p < np; p++) {
+            const fmm_entry_t pe = parts[p];
+            const int ec = swap ? pe.cols : pe.rows;
+            const int es = swap ? pe.rows : pe.cols;
+            xs[p][t] = (c < ec && s < es) ? (double)X[pe.off + c + (long)s * ld] : 0.0;
+        }
+        if (c < Lc) {
+            for (int r = 0; r < R; r++) {
+                double v = 0.0;
+                for (int e = ptr[r]; e < ptr[r + 1]; e++) {
+                    const fmm_entry_t en = ent[e];
+                    v = fma(en.coef, xs[en.part][t], v);
+                }
+                S[r * plane + c + (long)s * ldo] = (T)v;
+            }
+        }
+    }
+}
+
+template <typename T, int NPMAX, int TB>
+int launch_parts(const fmmhip_presum_seg_t* g, hipStream_t st)
+{
+    const long nbc = (g->Lc + TB - 1) / TB;
+    long blocks = nbc * g->Ls;
+    if (blocks <= 0) return FMMHIP_SUCCESS;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((presum_parts_kernel<T, NPMAX, TB>), dim3((unsigned)blocks), dim3(TB), 0, st, (const T*)g->X,
+                       g->ld, g->ptr, g->ent, g->parts, g->np, g->R, g->swap, g->Lc, g->Ls, g->ldo, (T*)g->S);
+    return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+}
+
+template <typename T>
+int launch_parts_typed(const fmmhip_presum_seg_t* g, hipStream_t st)
+{
+    // LDS per block = NPMAX * TB * 8 bytes: 32 KB, 64 KB, 64 KB, 128 KB
+    if (g->np <= 16) return launch_parts<T, 16, 256>(g, st);
+    if (g->np <= 32) return launch_parts<T, 32, 256>(g, st);
+    if (g->np <= 64) return launch_parts<T, 64, 128>(g, st);
+    if (g->np <= 256) return launch_parts<T, 256, 64>(g, st);
+    return FMMHIP_ERR_UNSUPPORTED;
+}
+
 }  // namespace
+
+extern "C" int fmmhip_presum_seg(int is_double, const fmmhip_presum_seg_t* seg, void* stream)
+{
+    if (!seg || !seg->parts || seg->np <= 0) return FMMHIP_ERR_INVALID;
+    return is_double ? launch_parts_typed<double>(seg, (hipStream_t)stream)
+                     : launch_parts_typed<float>(seg, (hipStream_t)stream);
+}
 
 extern "C" int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff,
                                    int nblocks, void* stream)
