@@ -279,3 +279,26 @@ def test_large_integer_fmm_equals_classical_on_device():
     F.sync()
     assert F.fro_diff(C1, C2) == 0.0
     assert F.fro(C1) > 0
+
+
+@pytest.mark.parametrize("m,n,k,levels", [(2048, 2048, 2048, 1), (1001, 999, 1003, 1), (1024, 768, 1536, 2)])
+def test_reference_style_c_harness(tmp_path, m, n, k, levels):
+    """The reference's harness flow in C (tests/c/strassen_oapi.c) on the
+    plain configuration (222, C += A*B, 2048^3) and ragged shapes: host
+    objects, 222.txt, alpha = beta = 1; normalised error <= 1e-14 against
+    the classical product and 64 sampled entries against host dot products."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = tmp_path / "strassen_oapi"
+    subprocess.run(["gcc", "-std=c11", "-O2", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c", "strassen_oapi.c"), "-o", str(exe),
+                    "-L", os.path.dirname(F.LIB_PATH), "-lfmm_blis_amd", "-lm",
+                    f"-Wl,-rpath,{os.path.dirname(F.LIB_PATH)}"], check=True)
+    r = subprocess.run([str(exe), str(m), str(n), str(k), alg_path("222"), "2", str(levels)],
+                       capture_output=True, text=True, timeout=300)
+    say("harness:", r.stdout.strip(), r.stderr.strip())
+    assert r.returncode == 0
+    cols = r.stdout.split()
+    assert [int(c) for c in cols[:3]] == [m, n, k]
+    assert float(cols[6]) < 1e-14

@@ -252,3 +252,15 @@ def test_batch_create_without_device_fails():
 def test_server_create_without_device_fails():
     h = ctypes.c_void_p()
     assert F.lib().fmmhip_server_create(ctypes.byref(h), 1 << 30, 4) == 5       # FMMHIP_ERR_NODEVICE
+
+
+def test_reference_style_c_harness_compiles_and_links(tmp_path):
+    """The C counterpart of the reference's harness (test_strassen_oapi.c)
+    compiles against the header and links against the library."""
+    exe = tmp_path / "strassen_oapi"
+    subprocess.run(["gcc", "-std=c11", "-O2", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c", "strassen_oapi.c"), "-o", str(exe),
+                    "-L", os.path.dirname(F.LIB_PATH), "-lfmm_blis_amd", "-lm",
+                    f"-Wl,-rpath,{os.path.dirname(F.LIB_PATH)}"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
