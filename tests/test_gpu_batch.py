@@ -181,3 +181,26 @@ def test_batch_tuner_candidates_and_replan():
     b.run()
     F.sync()
     b.close()
+
+
+def test_tuner_calibration_on_device(tmp_path):
+    """Calibration times every candidate on the device and refits the cost
+    model; the fitted table round-trips through a file, and the refitted
+    model still picks an FMM candidate for FP64 16384^3 and classical for
+    small problems (the measured ordering of the candidates)."""
+    before = tmp_path / "before.txt"
+    F.tuner_save(str(before))
+    try:
+        F.tuner_calibrate()
+        after = tmp_path / "after.txt"
+        F.tuner_save(str(after))
+        text = after.read_text()
+        say(text)
+        effs = [float(l.split()[3]) for l in text.splitlines() if l.startswith("cand ")]
+        assert len(effs) == 6 and all(0.2 < e <= 0.98 for e in effs)
+        assert F.tuner_choose(True, 256, 256, 256)[0] == 0
+        assert F.tuner_choose(True, 16384, 16384, 16384)[0] in (1, 2)
+        F.tuner_load(str(after))
+        assert (tmp_path / "after.txt").read_text() == text
+    finally:
+        F.tuner_load(str(before))
