@@ -138,8 +138,12 @@ extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, lo
     return FMMHIP_SUCCESS;
 }
 
-// Times each candidate on a few square sizes with the library's launch
-// timing and refits eff (largest size), ovh (smallest size) and bw.
+// Times each candidate at the square size whose leaf fills the device with
+// whole waves of tiles (classical 4096^3, one level 8192^3, two levels
+// 16384^3: 1024 tiles of 128x128 each) with the library's launch timing,
+// and refits its sustained efficiency (at the fixed prologue cost ovh) and
+// the summation bandwidth.  Sizes whose leaf leaves the device partly idle
+// would fold the wave quantisation into the efficiency.
 extern "C" int fmmhip_tuner_calibrate(void* stream)
 {
     int rc = fmmhip_init();
@@ -147,7 +151,7 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
     hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
     int cus = 256;
     fmmhip_device_cu_count(&cus);
-    const long big = 4096, small = 1024;
+    const long nmax = 4096L << (kCands - 1);
     TunerParams tp;
     {
         std::lock_guard<std::mutex> lk(g_tuner_mu);
@@ -155,81 +159,60 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
     }
     tp.cus = cus;
     double bw_sum = 0.0, bw_n = 0.0;
-    for (int dbl = 0; dbl < 2; dbl++) {
+    for (int dbl = 0; dbl < 2 && !rc; dbl++) {
         const size_t esz = dbl ? 8 : 4;
         void *A = nullptr, *B = nullptr, *C = nullptr;
-        if (fmmhip_malloc(&A, esz * big * big) || fmmhip_malloc(&B, esz * big * big) ||
-            fmmhip_malloc(&C, esz * big * big)) {
+        if (fmmhip_malloc(&A, esz * nmax * nmax) || fmmhip_malloc(&B, esz * nmax * nmax) ||
+            fmmhip_malloc(&C, esz * nmax * nmax)) {
             fmmhip_free(A); fmmhip_free(B); fmmhip_free(C);
             return FMMHIP_ERR_NOMEM;
         }
         if (dbl) {
-            fmmhip_fill_uniform((double*)A, big, big, big, 7, 0, st);
-            fmmhip_fill_uniform((double*)B, big, big, big, 7, 1, st);
-            fmmhip_fill_uniform((double*)C, big, big, big, 7, 2, st);
+            fmmhip_fill_uniform((double*)A, nmax, nmax, nmax, 7, 0, st);
+            fmmhip_fill_uniform((double*)B, nmax, nmax, nmax, 7, 1, st);
+            fmmhip_fill_uniform((double*)C, nmax, nmax, nmax, 7, 2, st);
         } else {
-            fmmhip_fill_uniform_f32((float*)A, big, big, big, 7, 0, st);
-            fmmhip_fill_uniform_f32((float*)B, big, big, big, 7, 1, st);
-            fmmhip_fill_uniform_f32((float*)C, big, big, big, 7, 2, st);
+            fmmhip_fill_uniform_f32((float*)A, nmax, nmax, nmax, 7, 0, st);
+            fmmhip_fill_uniform_f32((float*)B, nmax, nmax, nmax, 7, 1, st);
+            fmmhip_fill_uniform_f32((float*)C, nmax, nmax, nmax, 7, 2, st);
         }
-        for (int cand = 0; cand < kCands; cand++) {
+        for (int cand = 0; cand < kCands && !rc; cand++) {
             fmmhip_alg_t alg = cand_alg(cand);
             const int saved_pack = alg->pack;
             alg->pack = cand > 0 ? 1 : 0;
-            double meas[2] = {0, 0}, pres[2] = {0, 0};
-            const long sizes[2] = {big, small};
-            for (int si = 0; si < 2; si++) {
-                const long nn = sizes[si];
-                for (int rep = 0; rep < 3; rep++) {
-                    if (rep == 1) fmmhip_timing_enable(1);
-                    if (dbl)
-                        rc = fmmhip_dgemm_fmm(alg, FMMHIP_LEAF_NATIVE, nn, nn, nn, 1.0, (double*)A, 1, big,
-                                              (double*)B, 1, big, 1.0, (double*)C, 1, big, st);
-                    else
-                        rc = fmmhip_sgemm_fmm(alg, nn, nn, nn, 1.0f, (float*)A, 1, big, (float*)B, 1, big, 1.0f,
-                                              (float*)C, 1, big, st);
-                    if (rc) break;
-                }
-                fmmhip_timing_enable(0);
-                long nl = 0;
-                double kms = 0, pms = 0, mx = 0;
-                fmmhip_timing_read(&nl, &kms, &pms, &mx);
-                if (rc || nl <= 0) break;
-                meas[si] = kms / nl;
-                pres[si] = pms / nl;
+            const long nn = 4096L << cand;
+            for (int rep = 0; rep < 3 && !rc; rep++) {
+                if (rep == 1) fmmhip_timing_enable(1);
+                if (dbl)
+                    rc = fmmhip_dgemm_fmm(alg, FMMHIP_LEAF_NATIVE, nn, nn, nn, 1.0, (double*)A, 1, nmax, (double*)B,
+                                          1, nmax, 1.0, (double*)C, 1, nmax, st);
+                else
+                    rc = fmmhip_sgemm_fmm(alg, nn, nn, nn, 1.0f, (float*)A, 1, nmax, (float*)B, 1, nmax, 1.0f,
+                                          (float*)C, 1, nmax, st);
             }
+            fmmhip_timing_enable(0);
             alg->pack = saved_pack;
-            if (rc) break;
+            long nl = 0;
+            double kms = 0, pms = 0, mx = 0;
+            fmmhip_timing_read(&nl, &kms, &pms, &mx);
+            if (rc || nl <= 0) break;
             const CandShape& c = kShape[cand];
-            // eff from the big size (kernel only); ovh from the small one
-            const double lb = (double)big / c.mt, ls = (double)small / c.mt;
-            const double flb = 2.0 * c.R * lb * lb * lb, fls = 2.0 * c.R * ls * ls * ls;
-            const double nkb = std::ceil(lb / fmmk::BK), nks = std::ceil(ls / fmmk::BK);
-            const double tiles_s = std::ceil(ls / fmmk::BM) * std::ceil(ls / fmmk::BN);
-            const double q_s = tiles_s / (std::ceil(tiles_s / (2.0 * cus)) * 2.0 * cus);
-            const double tiles_b = std::ceil(lb / fmmk::BM) * std::ceil(lb / fmmk::BN);
-            const double q_b = tiles_b / (std::ceil(tiles_b / (2.0 * cus)) * 2.0 * cus);
-            const double eb = flb / (kPeak[dbl] * meas[0] * 1e-3) / q_b;   // = eff*nkb/(nkb+ovh)
-            const double es = fls / (kPeak[dbl] * meas[1] * 1e-3) / q_s;
-            // solve eff*nk/(nk+ovh) = e at both sizes
-            double ovh = 4.0, eff = eb;
-            if (es > 0 && eb > 0 && es < eb) {
-                ovh = (eb - es) / (es / nks - eb / nkb);
-                if (!(ovh > 0) || ovh > 64) ovh = 4.0;
-                eff = eb * (nkb + ovh) / nkb;
-            }
+            const double leaf = (double)nn / c.mt;
+            const double fl = 2.0 * c.R * leaf * leaf * leaf;
+            const double nk = std::ceil(leaf / fmmk::BK);
+            const double e = fl / (kPeak[dbl] * (kms / nl) * 1e-3);     // = eff * nk / (nk + ovh)
+            double eff = e * (nk + tp.ovh[dbl][cand]) / nk;
             if (eff > 0.98) eff = 0.98;
-            tp.eff[dbl][cand] = eff;
-            tp.ovh[dbl][cand] = ovh;
-            if (cand > 0 && pres[0] > 0) {
-                const double bytes = (double)esz * (2.0 * big * big + 2.0 * c.R * lb * lb);
-                bw_sum += bytes / (pres[0] * 1e-3);
+            if (eff > 0.05) tp.eff[dbl][cand] = eff;
+            if (cand > 0 && pms > 0) {
+                const double bytes = (double)esz * (2.0 * nn * nn + 2.0 * c.R * leaf * leaf);
+                bw_sum += bytes / (pms / nl * 1e-3);
                 bw_n += 1;
             }
         }
         fmmhip_free(A); fmmhip_free(B); fmmhip_free(C);
-        if (rc) return rc;
     }
+    if (rc) return rc;
     if (bw_n > 0) tp.bw = bw_sum / bw_n;
     std::lock_guard<std::mutex> lk(g_tuner_mu);
     g_tp = tp;
