@@ -249,6 +249,32 @@ def test_classical_object_entry_float():
     assert err < 1e-7
 
 
+@pytest.mark.parametrize("name", ["222", "333"])
+def test_dropin_float_objects_fmm(name):
+    """bli_strassen_ab_ex on float objects (FP32 FMM through the object API):
+    small-integer inputs are exact in FP32, so the result equals the oracle's
+    bit for bit; uniform inputs within 1e-6 of the FP64 product of the
+    FP32-rounded inputs."""
+    m, n, k = 131, 97, 203
+    f = fmm(name)
+    og = O.parse(alg_path(name))
+    A, B, C = O.integers(m, k, 8, 0, 2), O.integers(k, n, 8, 1, 2), O.integers(m, n, 8, 2, 2)
+    want = O.fmm_dgemm(og, 1.0, A, B, 1.0, C.copy(order="F")).astype(np.float32)
+    As, Bs, Cs = (np.asfortranarray(X.astype(np.float32)) for X in (A, B, C))
+    one, s1 = F.obj_scalar(1.0)
+    F.lib().bli_strassen_ab_ex(one, F.obj_attach(As, m, k, 1, m, F.BLIS_FLOAT),
+                               F.obj_attach(Bs, k, n, 1, k, F.BLIS_FLOAT), one,
+                               F.obj_attach(Cs, m, n, 1, m, F.BLIS_FLOAT), f)
+    assert np.array_equal(Cs, want)
+    U, V, W = O.uniform_f32(m, k, 9, 0), O.uniform_f32(k, n, 9, 1), O.uniform_f32(m, n, 9, 2)
+    Wc = W.copy(order="F")
+    F.lib().bli_strassen_ab_ex(one, F.obj_attach(U, m, k, 1, m, F.BLIS_FLOAT),
+                               F.obj_attach(V, k, n, 1, k, F.BLIS_FLOAT), one,
+                               F.obj_attach(Wc, m, n, 1, m, F.BLIS_FLOAT), f)
+    ref = W.astype(np.float64) + U.astype(np.float64) @ V.astype(np.float64)
+    assert O.normalized_error(Wc.astype(np.float64), ref, U.astype(np.float64), V.astype(np.float64)) < 1e-6
+
+
 @pytest.mark.parametrize("levels", [1, 2])
 def test_fp32_leaf_and_sgemm_fmm(levels):
     m = n = k = 512
