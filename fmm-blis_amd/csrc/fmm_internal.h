@@ -41,6 +41,7 @@ struct fmmhip_alg_s {
     int  shape;                 /* kernel launch shape, -1 = chosen per call */
     int  pack;                  /* 0 fused, 1 pre-summed workspace, -1 = chosen per call */
     int  keep_all;              /* never evict cached descriptors (owners that capture graphs) */
+    int  split;                 /* product groups per tile: -1 chosen per call, 1 (none), 2, 4, 8 */
     void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
 };
 
@@ -72,6 +73,13 @@ int fmmhip_scale_s(float* C, long m, long n, long rsC, long csC, float beta, voi
 int fmmhip_pointer_is_device(const void* p);
 /* The library's default stream (created on first use). */
 void* fmmhip_default_stream(void);
+/* Split-product epilogue helpers (fmmhip_runtime.hip): zero G temporaries,
+ * then C := beta*C + sum_g T_g over an m x n matrix with strides (rs, cs);
+ * each temporary spans `elems` elements with C's strides. */
+int fmmhip_split_zero(int is_double, void* T, long elems, int groups, void* stream);
+int fmmhip_split_reduce(int is_double, void* C, long m, long n, long rs, long cs, double beta, const void* T,
+                        long elems, int groups, void* stream);
+
 /* One operand of one problem for the batched summation stage. */
 typedef struct fmmhip_presum_seg_s {
     const void* X;              /* operand */

@@ -258,13 +258,25 @@ static int finish_alg(struct fmmhip_alg_s* g)
     if ((rc = build_csr(g->U, g->mt * g->kt, g->R, &g->u_ptr, &g->u_part, &g->u_coef, &g->nnz_u, &g->max_nnz_u))) return rc;
     if ((rc = build_csr(g->V, g->kt * g->nt, g->R, &g->v_ptr, &g->v_part, &g->v_coef, &g->nnz_v, &g->max_nnz_v))) return rc;
     if ((rc = build_csr(g->W, g->mt * g->nt, g->R, &g->w_ptr, &g->w_part, &g->w_coef, &g->nnz_w, &g->max_nnz_w))) return rc;
-    /* first product that writes each C part applies beta */
+    /* First writer of each C part: bit b of w_first[e] is set when entry e
+     * is the first write of its part among the products of its group, the
+     * products being split into 2^b contiguous groups [g*R/G, (g+1)*R/G)
+     * (G = 1, 2, 4, 8).  Bit 0 (one group) is the first product overall,
+     * which applies beta. */
     g->w_first = (int*)calloc((size_t)(g->nnz_w + 1), sizeof(int));
     char* seen = (char*)calloc((size_t)(g->mt * g->nt), 1);
     if (!g->w_first || !seen) { free(seen); return FMMHIP_ERR_NOMEM; }
-    for (long e = 0; e < g->nnz_w; e++) {
-        int p = g->w_part[e];
-        if (!seen[p]) { seen[p] = 1; g->w_first[e] = 1; }
+    for (int b = 0; b < 4; b++) {
+        const int G = 1 << b;
+        for (int grp = 0; grp < G; grp++) {
+            memset(seen, 0, (size_t)(g->mt * g->nt));
+            const int r0 = (int)((long)grp * g->R / G), r1 = (int)((long)(grp + 1) * g->R / G);
+            for (int r = r0; r < r1; r++)
+                for (int e = g->w_ptr[r]; e < g->w_ptr[r + 1]; e++) {
+                    int p = g->w_part[e];
+                    if (!seen[p]) { seen[p] = 1; g->w_first[e] |= 1 << b; }
+                }
+        }
     }
     free(seen);
     return FMMHIP_SUCCESS;
@@ -301,6 +313,7 @@ int fmmhip_alg_create(fmmhip_alg_t* out, const fmm_t* algs, int nlevels)
     if (!g) return FMMHIP_ERR_NOMEM;
     g->levels = nlevels;
     g->shape = -1;
+    g->split = -1;
     g->pack = -1;
     /* start from level 0 and fold the next levels in as inner blocks */
     int cm = algs[0].m_tilde, ck = algs[0].k_tilde, cn = algs[0].n_tilde, cR = algs[0].R;

@@ -403,7 +403,7 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         }
         b->est_ms += est;
         rc = prepare_call(b->algs[p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1, p.lda,
-                          p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? 1 : 0, b->stream, &b->prep[i]);
+                          p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? 1 : 0, 1, b->stream, &b->prep[i]);
         // the batched kernel is instantiated for column-major operands only
         if (!rc && !b->prep[i].nothing && !b->prep[i].scale_only &&
             !(b->prep[i].a_mn && !b->prep[i].b_mn && b->prep[i].c_col))

@@ -360,7 +360,7 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
                 for (int e = a.wptr[r]; e < a.wptr[r + 1]; e++) {
                     const fmm_entry_t we = a.ent[e];
                     const double lam = a.alpha * we.coef;
-                    const bool first = we.first != 0;
+                    const bool first = (we.first & 1) != 0;      // first writer over all products
                     double* ct = a.C + we.off + (long)(m0 + er0) * a.rsC + (long)(n0 + ec0) * a.csC;
                     const int erows = we.rows - m0 - er0, ecols = we.cols - n0 - ec0;
 #pragma unroll

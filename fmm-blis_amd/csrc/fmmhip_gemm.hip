@@ -3,7 +3,9 @@
 // fmmhip_{d,s}gemm{,_fmm} entry points of the C ABI.  The kernel itself is
 // in fmmhip_kernel.h, instantiated per datatype in fmmhip_gemm_*.hip.
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <mutex>
+#include <algorithm>
 #include <vector>
 #include <string.h>
 #include <stdlib.h>
@@ -245,6 +247,7 @@ bool pairs_aligned(const void* base, size_t esz, long ld, const fmm_entry_t* ent
 // launch stream around the summation stage and around the MFMA kernel.
 struct TimedCall {
     hipEvent_t e0, e1, e2;      // before summation, before kernel, after kernel
+    hipEvent_t e3;              // after the split reduction (counted with the summation), or null
     bool presum;
 };
 std::mutex g_tm_mu;
@@ -265,9 +268,33 @@ constexpr double kWsFractionOfFree = 0.5;
 
 }  // namespace
 
+// Product groups per tile for a grid of `tiles` workgroups: the power of
+// two (<= 8, <= R) that fills whole waves of the device's workgroup slots
+// best; ties go to fewer groups (each group adds a temporary of C).
+int split_groups(int requested, int R, long tiles)
+{
+    if (R < 2 || requested == 1) return 1;
+    if (requested > 1) {
+        int g = std::min(requested, 8);
+        while (g > R) g /= 2;
+        return g;
+    }
+    int cus = 256;
+    fmmhip_device_cu_count(&cus);
+    const double slots = 2.0 * cus;
+    int best = 1;
+    double best_eff = 0.0;
+    for (int g = 1; g <= 8 && g <= R; g *= 2) {
+        const double items = (double)tiles * g;
+        const double eff = items / (std::ceil(items / slots) * slots);
+        if (eff > best_eff + 0.02) { best = g; best_eff = eff; }
+    }
+    return best;
+}
+
 int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
-                 hipStream_t st, Prepared* P, DescSource* src)
+                 int split, hipStream_t st, Prepared* P, DescSource* src)
 {
     memset(P, 0, sizeof(*P));
     P->kind = kind;
@@ -275,9 +302,9 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     if (!alg || m < 0 || n < 0 || k < 0) return FMMHIP_ERR_INVALID;
     if (m == 0 || n == 0) { P->nothing = true; return FMMHIP_SUCCESS; }
     if (!C) return FMMHIP_ERR_INVALID;
+    P->C = C; P->m = m; P->n = n; P->rsC = rsC; P->csC = csC; P->beta = beta;
     if (k == 0 || alpha == 0.0) {
         P->scale_only = true;
-        P->C = C; P->m = m; P->n = n; P->rsC = rsC; P->csC = csC; P->beta = beta;
         return FMMHIP_SUCCESS;
     }
     if (!A || !B) return FMMHIP_ERR_INVALID;
@@ -387,16 +414,30 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     P->a_mn = a_mn;
     P->b_mn = b_mn;
     P->c_col = rsC == 1;
-    P->nwg = (long)ka.tiles_m * ka.tiles_n;
+    const long tiles = (long)ka.tiles_m * ka.tiles_n;
+    // product split: group g of every tile scatters into temporary g
+    ka.groups = split_groups(split, alg->R, tiles);
+    ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
+    ka.tmp = nullptr;
+    ka.tmp_elems = 0;
+    if (ka.groups > 1) {
+        ka.tmp_elems = ((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L;
+        const size_t base = P->presum ? ((P->ws_bytes + 255) & ~(size_t)255) : 0;
+        P->tmp_off = base;
+        P->ws_bytes = base + esz * (size_t)ka.tmp_elems * (size_t)ka.groups;
+        ka.tmp = (void*)(uintptr_t)base;
+    }
+    P->nwg = tiles * ka.groups;
     if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
-    if (P->nwg <= 0) P->nothing = true;
+    if (tiles <= 0) P->nothing = true;
     return FMMHIP_SUCCESS;
 }
 
 void bind_workspace(Prepared* P, void* ws)
 {
-    if (!P->presum) return;
     char* base = (char*)ws;
+    if (P->ka.groups > 1) P->ka.tmp = base + P->tmp_off;
+    if (!P->presum) return;
     P->segA.S = base + (uintptr_t)P->segA.S;
     P->segB.S = base + (uintptr_t)P->segB.S;
     P->ka.A = P->segA.S;
@@ -418,11 +459,15 @@ int launch_prepared(const Prepared* P)
         std::lock_guard<std::mutex> lk(g_tm_mu);
         timed = g_tm_on;
     }
-    TimedCall tc{nullptr, nullptr, nullptr, P->presum};
+    TimedCall tc{nullptr, nullptr, nullptr, nullptr, P->presum};
     if (timed) tm_record(&tc.e0, st);
     int rc = FMMHIP_SUCCESS;
+    const bool dbl = P->kind != Kind::S32;
+    if (P->ka.groups > 1) {
+        rc = fmmhip_split_zero(dbl, P->ka.tmp, P->ka.tmp_elems, P->ka.groups, st);
+        if (rc) return rc;
+    }
     if (P->presum && P->segA.parts && P->segA.np <= kPartStagedMax && P->segB.np <= kPartStagedMax) {
-        const bool dbl = P->kind != Kind::S32;
         rc = fmmhip_presum_seg(dbl, &P->segA, st);
         if (!rc) rc = fmmhip_presum_seg(dbl, &P->segB, st);
         if (rc) return rc;
@@ -449,8 +494,12 @@ int launch_prepared(const Prepared* P)
         break;
     case Kind::S32: rc = fmmk_launch_s32(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
     }
+    if (timed) tm_record(&tc.e2, st);
+    if (!rc && P->ka.groups > 1)
+        rc = fmmhip_split_reduce(dbl, P->C, P->m, P->n, P->rsC, P->csC, P->beta, P->ka.tmp, P->ka.tmp_elems,
+                                 P->ka.groups, st);
     if (timed) {
-        tm_record(&tc.e2, st);
+        if (P->ka.groups > 1) tm_record(&tc.e3, st);
         std::lock_guard<std::mutex> lk(g_tm_mu);
         g_tm.push_back(tc);
     }
@@ -467,14 +516,15 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
     Prepared P;
     if (!alg) return FMMHIP_ERR_INVALID;
     const int pack = alg->pack;
-    rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, pack, st, &P);
+    rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, pack, alg->split, st,
+                      &P);
     if (rc) return rc;
-    if (P.presum) {
+    if (P.ws_bytes) {
         void* ws = workspace_get(st, P.ws_bytes);
         if (!ws) {
-            if (pack == 1) return FMMHIP_ERR_NOMEM;
-            // automatic choice falls back to the fused mode
-            rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, 0, st, &P);
+            if (pack == 1 || alg->split > 1) return FMMHIP_ERR_NOMEM;
+            // automatic choices fall back to the fused mode without split
+            rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, 0, 1, st, &P);
             if (rc) return rc;
         } else {
             bind_workspace(&P, ws);
@@ -562,7 +612,7 @@ void timing_record(hipEvent_t* ev, hipStream_t st) { tm_record(ev, st); }
 void timing_push(hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, bool presum)
 {
     std::lock_guard<std::mutex> lk(g_tm_mu);
-    g_tm.push_back(TimedCall{e0, e1, e2, presum});
+    g_tm.push_back(TimedCall{e0, e1, e2, nullptr, presum});
 }
 
 fmmhip_alg_t classical_alg()
@@ -578,6 +628,13 @@ extern "C" int fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape)
 {
     if (!alg || shape < -1 || shape > 1) return FMMHIP_ERR_INVALID;
     alg->shape = shape;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_alg_set_split(fmmhip_alg_t alg, int groups)
+{
+    if (!alg || !(groups == -1 || groups == 1 || groups == 2 || groups == 4 || groups == 8)) return FMMHIP_ERR_INVALID;
+    alg->split = groups;
     return FMMHIP_SUCCESS;
 }
 
@@ -605,17 +662,20 @@ extern "C" int fmmhip_timing_read(long* launches, double* kernel_ms, double* pre
     long n = 0;
     double k_ms = 0.0, p_ms = 0.0, k_max = 0.0;
     for (TimedCall& c : g_tm) {
-        float a = 0.f, b = 0.f;
-        if (c.e0 && c.e1 && c.e2 && hipEventSynchronize(c.e2) == hipSuccess &&
-            hipEventElapsedTime(&a, c.e0, c.e1) == hipSuccess && hipEventElapsedTime(&b, c.e1, c.e2) == hipSuccess) {
+        float a = 0.f, b = 0.f, d = 0.f;
+        hipEvent_t last = c.e3 ? c.e3 : c.e2;
+        if (c.e0 && c.e1 && c.e2 && hipEventSynchronize(last) == hipSuccess &&
+            hipEventElapsedTime(&a, c.e0, c.e1) == hipSuccess && hipEventElapsedTime(&b, c.e1, c.e2) == hipSuccess &&
+            (!c.e3 || hipEventElapsedTime(&d, c.e2, c.e3) == hipSuccess)) {
             n++;
-            p_ms += a;
+            p_ms += a + d;
             k_ms += b;
             if (b > k_max) k_max = b;
         }
         if (c.e0) (void)hipEventDestroy(c.e0);
         if (c.e1) (void)hipEventDestroy(c.e1);
         if (c.e2) (void)hipEventDestroy(c.e2);
+        if (c.e3) (void)hipEventDestroy(c.e3);
     }
     g_tm.clear();
     if (launches) *launches = n;

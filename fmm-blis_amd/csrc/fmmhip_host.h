@@ -22,8 +22,9 @@ struct Prepared {
     bool a_mn, b_mn, c_col;
     long nwg;
     bool presum;                 // pre-summed pack mode
-    size_t ws_bytes;             // its workspace
+    size_t ws_bytes;             // workspace: summation leaves, then the split temporaries
     fmmhip_presum_seg_t segA, segB;
+    size_t tmp_off;              // split temporaries: workspace offset (ka.groups > 1)
 };
 
 // Device coefficient lists of one call: the fused list (every nonzero part
@@ -52,10 +53,11 @@ size_t desc_bound(fmmhip_alg_t alg);
 int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, void* host, const void* dev,
               size_t cap, size_t* used, DescView* v);
 
-// Resolves one call (pack: 0 fused, 1 pre-summed, -1 automatic).
+// Resolves one call (pack: 0 fused, 1 pre-summed, -1 automatic; split:
+// product groups per tile, 1, 2, 4 or 8, -1 chosen from the tile count).
 int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
-                 hipStream_t st, Prepared* P, DescSource* src = nullptr);
+                 int split, hipStream_t st, Prepared* P, DescSource* src = nullptr);
 // Rebases a pre-summed call's workspace offsets onto `ws`.
 void bind_workspace(Prepared* P, void* ws);
 // Launches a prepared, bound call on its stream.

@@ -93,6 +93,13 @@ struct KArgs {
     int R;
     int tiles_m, tiles_n;
     int va, vb;                  // pairs of A / B are 16-byte aligned (vector loads allowed)
+    // Product split: `groups` (1, 2, 4, 8) workgroups per tile, group g
+    // taking products [g*R/groups, (g+1)*R/groups) and scattering into its
+    // own temporary tmp + g*tmp_elems (C's strides; zeroed beforehand, reduced
+    // into C afterwards), first writers from bit gbit of the entries' flags.
+    int groups, gbit;
+    void* tmp;
+    long tmp_elems;
     const int* ptr;              // u, v, w CSR pointers, 3*(R+1)
     const fmm_entry_t* ent;
     double alpha, beta;
@@ -282,7 +289,7 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
 // body of both the one-problem kernel and the batched persistent kernel.
 // sA/sB are the workgroup's double-buffered LDS tiles.
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
-__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*sA)[lds_tile<TL>()],
+__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int grp, TL (*sA)[lds_tile<TL>()],
                                           TL (*sB)[lds_tile<TL>()])
 {
     constexpr int WAVES_M = Cfg<SHAPE>::WAVES_M;
@@ -299,8 +306,10 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*s
 
     const TI* __restrict__ Ab = (const TI*)a.A;
     const TI* __restrict__ Bb = (const TI*)a.B;
-    TO* __restrict__ Cb = (TO*)a.C;
     const int R = a.R;
+    const bool split = a.groups > 1;
+    TO* __restrict__ Cb = split ? (TO*)a.tmp + (long)grp * a.tmp_elems : (TO*)a.C;
+    const int r_begin = (int)((long)grp * R / a.groups), r_end = (int)((long)(grp + 1) * R / a.groups);
     const int* __restrict__ uptr = a.ptr;
     const int* __restrict__ vptr = a.ptr + (R + 1);
     const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
@@ -319,7 +328,7 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*s
     stB.init(tid, ldb);
 
 
-    for (int r = 0; r < R; r++) {
+    for (int r = r_begin; r < r_end; r++) {
         const int u0 = uptr[r], nu = uptr[r + 1] - u0;
         const int v0 = vptr[r], nv = vptr[r + 1] - v0;
         const int ng = nu > nv ? nu : nv;   // load groups: part g of A and part g of B
@@ -439,8 +448,8 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, TL (*s
         for (int e = w0; e < w1; e++) {
             const fmm_entry_t we = ent[e];
             const double lam = a.alpha * we.coef;
-            const bool first = we.first != 0;
-            const double beta = a.beta;
+            const bool first = ((we.first >> a.gbit) & 1) != 0;
+            const double beta = split ? 0.0 : a.beta;      // temporaries start at zero
             const long rsC = C_COL ? 1 : a.rsC;
             const int rs32 = (int)rsC, cs32 = (int)a.csC;
             // lane's element (i, j, q) is at tile row er0 + 16 i + ROW_REG q, col ec0 + 16 j
@@ -474,9 +483,12 @@ void fmm_gemm_kernel(KArgs a)
 {
     __shared__ TL sA[2][lds_tile<TL>()];
     __shared__ TL sB[2][lds_tile<TL>()];
+    // blocks [g*tiles, (g+1)*tiles) are product group g
+    const int tiles = a.tiles_m * a.tiles_n;
+    const int grp = (int)blockIdx.x / tiles;
     int tm, tn;
-    tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
-    gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL>(a, tm, tn, sA, sB);
+    tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
+    gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL>(a, tm, tn, grp, sA, sB);
 }
 
 // Batched persistent kernel: a fixed grid of workgroups takes work items
@@ -508,7 +520,7 @@ void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restri
         const BatchItem it = items[w];
         const KArgs a = probs[it.problem];
         const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
-        gemm_tile<TI, TL, TO, SHAPE, VEC, true, false, true>(a, tm, tn, sA, sB);
+        gemm_tile<TI, TL, TO, SHAPE, VEC, true, false, true>(a, tm, tn, 0, sA, sB);
         __syncthreads();                       // LDS free before the next item's prologue
     }
 }

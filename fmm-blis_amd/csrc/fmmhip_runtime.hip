@@ -313,6 +313,61 @@ extern "C" int fmmhip_scale_s(float* C, long m, long n, long rs, long cs, float 
     return 0;
 }
 
+// ---- product split: temporaries and their reduction -------------------------
+
+// Zeroes `n16` 16-byte words (the G temporaries, contiguous and aligned).
+__global__ void zero16_kernel(uint4* __restrict__ T, long n16)
+{
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x) T[i] = z;
+}
+
+// C := beta*C + T_0 + T_1 + ... + T_{G-1} element by element, in that order
+// (beta == 0 does not read C).  The walk's fast index is the dimension
+// with the smaller stride, so both C and the temporaries coalesce.
+template <typename T>
+__global__ void split_reduce_kernel(T* __restrict__ C, long inner, long outer, long si, long so, T beta,
+                                    const T* __restrict__ tmp, long elems, int groups)
+{
+    const long total = inner * outer;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        const long o = idx / inner, i = idx - o * inner;
+        const long off = i * si + o * so;
+        T v = beta == (T)0 ? (T)0 : beta * C[off];
+        for (int g = 0; g < groups; g++) v += tmp[(long)g * elems + off];
+        C[off] = v;
+    }
+}
+
+extern "C" int fmmhip_split_zero(int is_double, void* T, long elems, int groups, void* s)
+{
+    INIT_OR_RETURN();
+    const long bytes = elems * groups * (is_double ? 8 : 4);
+    if (bytes <= 0) return 0;
+    if ((bytes & 15) || ((uintptr_t)T & 15)) return FMMHIP_ERR_INVALID;
+    const long n16 = bytes / 16;
+    hipLaunchKernelGGL(zero16_kernel, dim3(grid_for(n16)), dim3(256), 0, S(s), (uint4*)T, n16);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int fmmhip_split_reduce(int is_double, void* C, long m, long n, long rs, long cs, double beta,
+                                   const void* T, long elems, int groups, void* s)
+{
+    INIT_OR_RETURN();
+    if (m <= 0 || n <= 0) return 0;
+    const bool by_row = rs <= cs;
+    const long inner = by_row ? m : n, outer = by_row ? n : m, si = by_row ? rs : cs, so = by_row ? cs : rs;
+    if (is_double)
+        hipLaunchKernelGGL(split_reduce_kernel<double>, dim3(grid_for(m * n)), dim3(256), 0, S(s), (double*)C, inner,
+                           outer, si, so, beta, (const double*)T, elems, groups);
+    else
+        hipLaunchKernelGGL(split_reduce_kernel<float>, dim3(grid_for(m * n)), dim3(256), 0, S(s), (float*)C, inner,
+                           outer, si, so, (float)beta, (const float*)T, elems, groups);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
 // ---- Frobenius norms (FP64, deterministic two-stage reduction) -------------
 
 template <bool DIFF>

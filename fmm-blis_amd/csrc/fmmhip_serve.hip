@@ -200,14 +200,14 @@ bool fmmhip_server_s::try_admit(Req& q, Slot& sl, int si)
     src.cap = list_cap;
     Prepared P;
     int rc = prepare_call(alg, r.is_double ? Kind::D64 : Kind::S32, r.m, r.n, r.k, r.alpha, r.A, 1, r.lda, r.B, 1,
-                          r.ldb, r.beta, C, 1, ldc, q.cand > 0 ? 1 : 0, sl.st, &P, &src);
+                          r.ldb, r.beta, C, 1, ldc, q.cand > 0 ? 1 : 0, alg->split, sl.st, &P, &src);
     if (rc) {
         arena.release(off1, b1);
         finish(q, rc);
         return true;
     }
     size_t off2 = kNoBlock;
-    if (P.presum) {
+    if (P.ws_bytes) {
         if (((b1 + 255) & ~(size_t)255) + ((P.ws_bytes + 255) & ~(size_t)255) > arena.cap) {
             arena.release(off1, b1);
             finish(q, FMMHIP_ERR_NOMEM);
@@ -222,7 +222,7 @@ bool fmmhip_server_s::try_admit(Req& q, Slot& sl, int si)
     q.off1 = off1;
     q.bytes1 = b1;
     q.off2 = off2;
-    q.bytes2 = P.presum ? P.ws_bytes : 0;
+    q.bytes2 = P.ws_bytes;
     q.C = C;
     q.ldc = ldc;
     q.own_c = own_c;

@@ -117,6 +117,7 @@ _SIGS = {
     "fmmhip_alg_brent_violations": (_L, [_P]),
     "fmmhip_alg_set_shape": (_I, [_P, _I]),
     "fmmhip_alg_set_pack": (_I, [_P, _I]),
+    "fmmhip_alg_set_split": (_I, [_P, _I]),
     "fmmhip_workspace_release": (_I, []),
     "fmmhip_timing_enable": (_I, [_I]),
     "fmmhip_timing_read": (_I, [ctypes.POINTER(_L)] + [ctypes.POINTER(_D)] * 3),
@@ -236,6 +237,10 @@ class Alg:
 
     def set_pack(self, pack):
         check(lib().fmmhip_alg_set_pack(self.h, pack), "alg_set_pack")
+
+    def set_split(self, groups):
+        """Product groups per C tile: 1, 2, 4, 8, or -1 (chosen per call)."""
+        check(lib().fmmhip_alg_set_split(self.h, groups), "alg_set_split")
 
     def close(self):
         if self.h:

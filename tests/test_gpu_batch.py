@@ -84,10 +84,13 @@ def expected(p):
 
 
 def alg_for(cand):
+    """The single-call algorithm the batch runs for `cand` (pre-summed, no
+    product split)."""
     if cand == 0:
         return F.Alg(classical=True)
     a = F.Alg([F.strassen()] * cand)
     a.set_pack(1)
+    a.set_split(1)
     return a
 
 

@@ -120,6 +120,17 @@ def test_alg_compose_matches_oracle(names):
         F.free_fmm(f)
 
 
+def test_alg_set_split_values():
+    """Product split per tile: -1 (per call), 1, 2, 4 or 8; others rejected."""
+    a = F.Alg([F.strassen()] * 2)
+    for g in (-1, 1, 2, 4, 8):
+        a.set_split(g)
+    for g in (0, 3, 16, -2):
+        with pytest.raises(F.FmmError):
+            a.set_split(g)
+    a.close()
+
+
 def test_alg_create_rejects_invalid_table(tmp_path):
     text = open(alg_path("222")).read().replace("1 0 1 0 1 -1 0", "1 0 1 0 1 1 0", 1)
     p = tmp_path / "bad222.txt"
