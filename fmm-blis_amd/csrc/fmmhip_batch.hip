@@ -30,7 +30,8 @@ namespace {
 // F_leaf the executed leaf flops, P the MFMA peak of the datatype, eff the
 // kernel's sustained fraction of peak, nk the 16-wide k steps per product,
 // ovh the per-product prologue/epilogue cost in k steps, quant the
-// wave-quantisation loss of a lone launch (1 inside a batch), bytes_sum the
+// wave-quantisation loss of a lone launch without product split (1 inside
+// a batch), bytes_sum the
 // summation stage's traffic and bw its bandwidth.
 constexpr int kCands = 3;
 struct TunerParams {
@@ -178,8 +179,9 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
         }
         for (int cand = 0; cand < kCands && !rc; cand++) {
             fmmhip_alg_t alg = cand_alg(cand);
-            const int saved_pack = alg->pack;
+            const int saved_pack = alg->pack, saved_split = alg->split;
             alg->pack = cand > 0 ? 1 : 0;
+            alg->split = 1;             // as the batch and the server run them
             const long nn = 4096L << cand;
             for (int rep = 0; rep < 3 && !rc; rep++) {
                 if (rep == 1) fmmhip_timing_enable(1);
@@ -192,6 +194,7 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
             }
             fmmhip_timing_enable(0);
             alg->pack = saved_pack;
+            alg->split = saved_split;
             long nl = 0;
             double kms = 0, pms = 0, mx = 0;
             fmmhip_timing_read(&nl, &kms, &pms, &mx);
