@@ -285,6 +285,38 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     tn = in / gsz;
 }
 
+// One 16-column block (column block j of the wave's accumulators) of the W
+// epilogue for one C part: C := (first ? beta*C : C) + lam*acc.  Every read
+// of C is issued before any write, so the reads are in flight together, and
+// each write goes out from its own register.  CHECK clips to the part
+// (erows/ecols: the part's extent from the block's first row/column).
+template <bool CHECK, int FI, int ROW_REG, bool C_COL, typename TO, typename ACC, int FJ>
+__device__ __forceinline__ void epi_block(TO* __restrict__ cb, int rs32, int erows, int ecols,
+                                          const ACC (&acc)[FI][FJ], int j, double lam, bool first, double beta,
+                                          bool rd)
+{
+    double v[FI][4];
+#pragma unroll
+    for (int i = 0; i < FI; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int dr = 16 * i + ROW_REG * q;
+            const TO* p = cb + (C_COL ? dr : dr * rs32);
+            v[i][q] = (rd && (!CHECK || (dr < erows && 0 < ecols))) ? (double)*p : 0.0;
+        }
+#pragma unroll
+    for (int i = 0; i < FI; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[i][q] = fma(lam, (double)acc[i][j][q], first ? beta * v[i][q] : v[i][q]);
+#pragma unroll
+    for (int i = 0; i < FI; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int dr = 16 * i + ROW_REG * q;
+            if (!CHECK || (dr < erows && 0 < ecols)) cb[C_COL ? dr : dr * rs32] = (TO)v[i][q];
+        }
+}
+
 // One BM x BN tile position (tm, tn) of the leaf, all R products: the
 // body of both the one-problem kernel and the batched persistent kernel.
 // sA/sB are the workgroup's double-buffered LDS tiles.
@@ -344,8 +376,13 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
         // prologue: stage k-step 0 (all groups, one after another)
         {
             const bool ia = a_rows_in && BK <= a.min_k, ib = b_cols_in && BK <= a.min_k;
-            stA.template load_first<VEC>(Ab, ea0, lda, m0, 0, ia);
-            stB.template load_first<VEC>(Bb, eb0, ldb, n0, 0, ib);
+            if (ia && ib) {
+                stA.template load_first<VEC>(Ab, ea0, lda, m0, 0, true);
+                stB.template load_first<VEC>(Bb, eb0, ldb, n0, 0, true);
+            } else {
+                stA.template load_first<VEC>(Ab, ea0, lda, m0, 0, ia);
+                stB.template load_first<VEC>(Bb, eb0, ldb, n0, 0, ib);
+            }
             stA.scale_first(ea0.coef);
             stB.scale_first(eb0.coef);
             for (int g = 1; g < nu; g++) {
@@ -374,8 +411,15 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
             const bool kin = k0 + BK <= a.min_k;
             const bool ia = a_rows_in && kin, ib = b_cols_in && kin;
             if (more) {
-                stA.template load_first<VEC>(Ab, ea0, lda, m0, k0, ia);
-                stB.template load_first<VEC>(Bb, eb0, ldb, n0, k0, ib);
+                // both operands' first parts in one block when the tiles are
+                // interior (no per-operand branches between the loads)
+                if (ia && ib) {
+                    stA.template load_first<VEC>(Ab, ea0, lda, m0, k0, true);
+                    stB.template load_first<VEC>(Bb, eb0, ldb, n0, k0, true);
+                } else {
+                    stA.template load_first<VEC>(Ab, ea0, lda, m0, k0, ia);
+                    stB.template load_first<VEC>(Bb, eb0, ldb, n0, k0, ib);
+                }
                 if (MULTI) {
                     if (1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + 1], lda, m0, k0, ia);
                     if (1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + 1], ldb, n0, k0, ib);
@@ -455,22 +499,15 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
             // lane's element (i, j, q) is at tile row er0 + 16 i + ROW_REG q, col ec0 + 16 j
             TO* ct = Cb + we.off + (long)(m0 + er0) * rsC + (long)(n0 + ec0) * a.csC;
             const int erows = we.rows - m0 - er0, ecols = we.cols - n0 - ec0;
+            const bool rd = !first || beta != 0.0;          // uniform: C is read
 #pragma unroll
             for (int j = 0; j < FJ; j++) {
-#pragma unroll
-                for (int i = 0; i < FI; i++) {
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int dr = 16 * i + Leaf<TL>::ROW_REG * q, dc = 16 * j;
-                        if (c_in || (dr < erows && dc < ecols)) {
-                            TO* p = ct + (C_COL ? dr + dc * cs32 : dr * rs32 + dc * cs32);
-                            double old = 0.0;
-                            if (!first) old = (double)*p;
-                            else if (beta != 0.0) old = beta * (double)*p;
-                            *p = (TO)fma(lam, (double)acc[i][j][q], old);
-                        }
-                    }
-                }
+                TO* cb = ct + 16 * j * cs32;
+                if (c_in)
+                    epi_block<false, FI, Leaf<TL>::ROW_REG, C_COL>(cb, rs32, 0, 0, acc, j, lam, first, beta, rd);
+                else
+                    epi_block<true, FI, Leaf<TL>::ROW_REG, C_COL>(cb, rs32, erows, ecols - 16 * j, acc, j, lam, first,
+                                                                 beta, rd);
             }
         }
     }
