@@ -44,10 +44,10 @@ struct TunerParams {
 
 std::mutex g_tuner_mu;
 TunerParams g_tp = {
-    // defaults measured on MI355X (16384^3: classical 78%, pre-summed FMM kernel 72% of peak)
-    {{0.70, 0.66, 0.66}, {0.78, 0.72, 0.72}},
+    // defaults: fmmhip_tuner_calibrate on an MI355X (FP32 row, then FP64)
+    {{0.62, 0.66, 0.64}, {0.79, 0.83, 0.81}},
     {{4.0, 4.0, 4.0}, {4.0, 4.0, 4.0}},
-    2.3e12, 0.01, 256};
+    3.2e12, 0.01, 256};
 
 const double kPeak[2] = {157.3e12, 78.6e12};   // FP32, FP64 MFMA peaks (FLOP/s)
 
