@@ -427,6 +427,9 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
             }
             const TL* la = sA[cur];
             const TL* lb = sB[cur];
+            // LLVM's single-wave small-GEMM interleaving of the LDS reads
+            // with the MFMAs (+1.5% measured on the FP64 kernels)
+            __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
                 const int kk = 4 * ks + (lane >> 4);
