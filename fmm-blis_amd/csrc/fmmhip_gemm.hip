@@ -268,10 +268,13 @@ constexpr double kWsFractionOfFree = 0.5;
 
 }  // namespace
 
-// Product groups per tile for a grid of `tiles` workgroups: the power of
-// two (<= 8, <= R) that fills whole waves of the device's workgroup slots
-// best; ties go to fewer groups (each group adds a temporary of C).
-int split_groups(int requested, int R, long tiles)
+// Product groups per tile for a grid of `tiles` workgroups.  Time model
+// per choice G: the leaf flops at a sustained MFMA rate scaled by the
+// fraction of the last wave of workgroup slots (2 per CU) that is filled,
+// plus, for G > 1, the temporaries' traffic at HBM speed (G copies of C
+// zeroed, read back by the reduction, C read and written once).  G is a
+// power of two <= min(8, R); ties go to fewer groups.
+int split_groups(int requested, int R, long tiles, double leaf_flops, double c_bytes, bool fp64)
 {
     if (R < 2 || requested == 1) return 1;
     if (requested > 1) {
@@ -282,12 +285,16 @@ int split_groups(int requested, int R, long tiles)
     int cus = 256;
     fmmhip_device_cu_count(&cus);
     const double slots = 2.0 * cus;
+    const double rate = (fp64 ? 78.6e12 : 157.3e12) * 0.85;   // sustained MFMA FLOP/s
+    const double hbm = 4.0e12;                                // achieved bytes/s of the fill and reduction
     int best = 1;
-    double best_eff = 0.0;
+    double best_t = 0.0;
     for (int g = 1; g <= 8 && g <= R; g *= 2) {
         const double items = (double)tiles * g;
-        const double eff = items / (std::ceil(items / slots) * slots);
-        if (eff > best_eff + 0.02) { best = g; best_eff = eff; }
+        const double fill = items / (std::ceil(items / slots) * slots);
+        double t = leaf_flops / (rate * fill);
+        if (g > 1) t += c_bytes * (2.0 * g + 2.0) / hbm;
+        if (g == 1 || t < 0.98 * best_t) { best = g; best_t = t; }
     }
     return best;
 }
@@ -416,7 +423,8 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     P->c_col = rsC == 1;
     const long tiles = (long)ka.tiles_m * ka.tiles_n;
     // product split: group g of every tile scatters into temporary g
-    ka.groups = split_groups(split, alg->R, tiles);
+    ka.groups = split_groups(split, alg->R, tiles, 2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln,
+                             (double)esz * (double)m * (double)n, kind == Kind::D64);
     ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
     ka.tmp = nullptr;
     ka.tmp_elems = 0;
