@@ -183,6 +183,12 @@ struct GemmArgs {
     double alpha, beta;
     int tiles_m, tiles_n;
     int min_cm, min_cn;          // tiles inside [0,min_cm) x [0,min_cn) are inside every C part
+    // product split: `groups` workgroups per tile, group g taking products
+    // [g*R/groups, (g+1)*R/groups) into its zeroed temporary tmp + g*tmp_elems
+    // (C's strides), first writers from bit gbit of the entries' flags
+    int groups, gbit;
+    double* tmp;
+    long tmp_elems;
 };
 
 template <int WM, int WN, int FI, int FJ>
@@ -208,8 +214,14 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
     __shared__ u32x4 sB[2][G::B_CH];
     __shared__ int sE[2][BM + BN];
 
+    // blocks [g*tiles, (g+1)*tiles) are product group g
+    const int tiles = a.tiles_m * a.tiles_n;
+    const int grp = (int)blockIdx.x / tiles;
+    const int r_begin = (int)((long)grp * a.R / a.groups), r_end = (int)((long)(grp + 1) * a.R / a.groups);
+    double* const Cb = a.groups > 1 ? a.tmp + (long)grp * a.tmp_elems : a.C;
+    const double beta_c = a.groups > 1 ? 0.0 : a.beta;      // temporaries start at zero
     int tm, tn;
-    fmmk::tile_of(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    fmmk::tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % WM) * (FI * 16), wn = (wave / WM) * (FJ * 16);
@@ -260,7 +272,7 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
             for (int q = 0; q < 4; q++) acc[i][j][q] = 0.0;
         }
 
-    BF3_GLOAD(0, 0);
+    BF3_GLOAD(r_begin, 0);
     BF3_LSTORE(0, true, 0);
     __syncthreads();
 
@@ -271,11 +283,11 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
     const int rd_b = (wn + (lane & 15)) * 4 + (kc ^ (((lane & 15) >> 1) & 3));
     int st = 0;           // LDS stage of the current step
     int chunk = 0;        // chunks completed (exponent buffer parity)
-    for (int r = 0; r < a.R; r++) {
+    for (int r = r_begin; r < r_end; r++) {
         for (int kb = 0; kb < a.nkb; kb++) {
             int rn = r, kbn = kb + 1;
             if (kbn == a.nkb) { rn = r + 1; kbn = 0; }
-            const bool more = rn < a.R;
+            const bool more = rn < r_end;
             if (more) BF3_GLOAD(rn, kbn);
 
             // the plane products of the step
@@ -360,8 +372,8 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
                 for (int e = a.wptr[r]; e < a.wptr[r + 1]; e++) {
                     const fmm_entry_t we = a.ent[e];
                     const double lam = a.alpha * we.coef;
-                    const bool first = (we.first & 1) != 0;      // first writer over all products
-                    double* ct = a.C + we.off + (long)(m0 + er0) * a.rsC + (long)(n0 + ec0) * a.csC;
+                    const bool first = ((we.first >> a.gbit) & 1) != 0;   // first writer within the group
+                    double* ct = Cb + we.off + (long)(m0 + er0) * a.rsC + (long)(n0 + ec0) * a.csC;
                     const int erows = we.rows - m0 - er0, ecols = we.cols - n0 - ec0;
 #pragma unroll
                     for (int j = 0; j < FJ; j++)
@@ -374,7 +386,7 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
                                     double* p = ct + (long)dr * a.rsC + (long)dc * a.csC;
                                     double old = 0.0;
                                     if (!first) old = *p;
-                                    else if (a.beta != 0.0) old = a.beta * *p;
+                                    else if (beta_c != 0.0) old = beta_c * *p;
                                     *p = fma(lam, acc[i][j][q], old);
                                 }
                             }
@@ -439,7 +451,13 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const size_t expA = (size_t)R * nch * LpA * 4, expB = (size_t)R * nch * LpB * 4;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t oPB = al(planesA), oEA = oPB + al(planesB), oEB = oEA + al(expA);
-    const size_t bytes = oEB + al(expB);
+    const long tiles = (LpA / Cfg3::BM) * (LpB / Cfg3::BN);
+    // product split (bf16 MFMA rate: 8 or 6 plane products per leaf product)
+    const int groups = split_groups(alg->split, R, tiles, 2.0 * terms * R * (double)g.lm * (double)g.lk * (double)g.ln,
+                                    8.0 * (double)m * (double)n, false, 2.5e15, 1);   // one workgroup per CU (LDS)
+    const long tmp_elems = groups > 1 ? (((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L) : 0;
+    const size_t oT = oEB + al(expB);
+    const size_t bytes = oT + (size_t)groups * tmp_elems * 8;
     char* ws = (char*)workspace_for(st, bytes);
     if (!ws) return FMMHIP_ERR_NOMEM;
 
@@ -461,6 +479,7 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     if (csB == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gB, blk, 0, st, sb);
     else hipLaunchKernelGGL(bf3::split_kernel<false>, gB, blk, 0, st, sb);
     if (hipGetLastError() != hipSuccess) return FMMHIP_ERR_HIP;
+    if (groups > 1 && (rc = fmmhip_split_zero(1, ws + oT, tmp_elems, groups, st))) return rc;
     if (timed) timing_record(&e1, st);
 
     bf3::GemmArgs ga;
@@ -479,7 +498,11 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     ga.tiles_n = (int)(LpB / Cfg3::BN);
     ga.min_cm = (int)(m / alg->mt);
     ga.min_cn = (int)(n / alg->nt);
-    const long nwg = (long)ga.tiles_m * ga.tiles_n;
+    ga.groups = groups;
+    ga.gbit = groups == 8 ? 3 : groups == 4 ? 2 : groups == 2 ? 1 : 0;
+    ga.tmp = groups > 1 ? (double*)(ws + oT) : nullptr;
+    ga.tmp_elems = tmp_elems;
+    const long nwg = (long)ga.tiles_m * ga.tiles_n * groups;
     if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
     if (terms == 8)
         hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 8>), dim3((unsigned)nwg), dim3(Cfg3::NT), 0,
@@ -492,5 +515,6 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
         timing_record(&e2, st);
         timing_push(e0, e1, e2, true);
     }
+    if (groups > 1) return fmmhip_split_reduce(1, C, m, n, rsC, csC, beta, ws + oT, tmp_elems, groups, st);
     return FMMHIP_SUCCESS;
 }

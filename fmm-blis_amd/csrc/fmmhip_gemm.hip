@@ -268,13 +268,16 @@ constexpr double kWsFractionOfFree = 0.5;
 
 }  // namespace
 
-// Product groups per tile for a grid of `tiles` workgroups.  Time model
-// per choice G: the leaf flops at a sustained MFMA rate scaled by the
-// fraction of the last wave of workgroup slots (2 per CU) that is filled,
+// Product groups per tile for a grid of `tiles` workgroups (peak: the
+// leaf's MFMA peak in FLOP/s, 0 = FP64 or FP32 by `fp64`; wg_per_cu: the
+// kernel's resident workgroups per CU).  Time model per choice G: the leaf
+// flops at a sustained MFMA rate scaled by the fraction of the last wave
+// of workgroup slots that is filled,
 // plus, for G > 1, the temporaries' traffic at HBM speed (G copies of C
 // zeroed, read back by the reduction, C read and written once).  G is a
 // power of two <= min(8, R); ties go to fewer groups.
-int split_groups(int requested, int R, long tiles, double leaf_flops, double c_bytes, bool fp64)
+int split_groups(int requested, int R, long tiles, double leaf_flops, double c_bytes, bool fp64, double peak,
+                 int wg_per_cu)
 {
     if (R < 2 || requested == 1) return 1;
     if (requested > 1) {
@@ -284,8 +287,8 @@ int split_groups(int requested, int R, long tiles, double leaf_flops, double c_b
     }
     int cus = 256;
     fmmhip_device_cu_count(&cus);
-    const double slots = 2.0 * cus;
-    const double rate = (fp64 ? 78.6e12 : 157.3e12) * 0.85;   // sustained MFMA FLOP/s
+    const double slots = (double)wg_per_cu * cus;
+    const double rate = (peak > 0.0 ? peak : fp64 ? 78.6e12 : 157.3e12) * 0.85;   // sustained MFMA FLOP/s
     const double hbm = 4.0e12;                                // achieved bytes/s of the fill and reduction
     int best = 1;
     double best_t = 0.0;
@@ -424,7 +427,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     const long tiles = (long)ka.tiles_m * ka.tiles_n;
     // product split: group g of every tile scatters into temporary g
     ka.groups = split_groups(split, alg->R, tiles, 2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln,
-                             (double)esz * (double)m * (double)n, kind == Kind::D64);
+                             (double)esz * (double)m * (double)n, kind == Kind::D64, 0.0, 2);
     ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
     ka.tmp = nullptr;
     ka.tmp_elems = 0;

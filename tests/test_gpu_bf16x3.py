@@ -3,7 +3,7 @@ and FMMHIP_LEAF_BF16X3_6, six): FP64 operands split into three bf16 planes
 with per-(row, 256-k chunk) power-of-two scales, plane products on the bf16
 MFMA with FP32 accumulation, FP64 recombination and FP64 W write-back.
 
-Bars (stated per test):
+Bars (stated per test; the product split is covered by the last two tests):
   * small-integer inputs are split exactly and every chunk sum is an exact
     FP32 integer, so the result equals the oracle's bit for bit, for the
     classical leaf and for 1- and 2-level FMM, on ragged shapes;
@@ -138,3 +138,38 @@ def test_bf16x3_nan_row():
     assert np.isnan(got[17, :]).all()
     rest = np.delete(got, 17, axis=0)
     assert np.isfinite(rest).all()
+
+
+@pytest.mark.parametrize("name", ["222x2", "333x1"])
+def test_bf16x3_product_split_exact_integers(name):
+    """The product split (G workgroups per tile over contiguous product
+    ranges, zeroed temporaries, ordered reduction) keeps integer inputs exact
+    for every G, beta applied once and beta = 0 not reading C."""
+    alg, og = algs()[name]
+    for (m, n, k) in [(129, 130, 257), (300, 280, 600)]:
+        A, B, C = O.integers(m, k, 6, 0, 4), O.integers(k, n, 6, 1, 4), O.integers(m, n, 6, 2, 4)
+        for alpha, beta in [(1.0, 1.0), (-2.0, 0.5), (1.0, 0.0)]:
+            want = O.fmm_dgemm(og, alpha, A, B, beta, C.copy(order="F"))
+            Cn = C.copy(order="F")
+            if beta == 0.0:
+                Cn[:] = np.nan
+            for g in (1, 2, 4, 8):
+                alg.set_split(g)
+                say("bf16x3 split", name, g, m, n, k, alpha, beta)
+                got = run(alg, alpha, A, B, beta, Cn)
+                assert np.array_equal(got, want), (g, m, n, k, alpha, beta)
+    alg.set_split(-1)
+
+
+def test_bf16x3_product_split_random_repeatable():
+    alg, _ = algs()["222x2"]
+    m = n = k = 1024
+    A, B, C = O.uniform(m, k, 8, 0), O.uniform(k, n, 8, 1), O.uniform(m, n, 8, 2)
+    ref = O.dgemm(1.0, A, B, 1.0, C.copy(order="F"))
+    for g in (2, 8):
+        alg.set_split(g)
+        outs = [run(alg, 1.0, A, B, 1.0, C) for _ in range(2)]
+        err = O.normalized_error(outs[0], ref, A, B)
+        say(f"bf16x3 222x2 split {g} normalised error {err:.3e}")
+        assert err < TOL[(8, "222x2")]
+        assert np.array_equal(outs[0], outs[1])
