@@ -64,6 +64,8 @@ struct SplitArgs {
     int Lp, ldk;                 // padded rows, padded k
     u16* P;                      // planes
     int* E;                      // exponents
+    int R;                       // products
+    int nrc;                     // (row block, chunk) pairs: (Lp / SROWS) * (ldk / CHUNK)
 };
 
 __device__ __forceinline__ u16 bf16_exact(double d)   // |d| integer <= 256
@@ -78,31 +80,58 @@ __device__ __forceinline__ u16 bf16_rne(double d)
     return (u16)(u >> 16);
 }
 
-// Block: SROWS rows x one CHUNK of k of product blockIdx.z.
+// Block: SROWS rows x one CHUNK of k of one product.  Block ids are mapped
+// so that the R blocks of one (row block, chunk) run back to back on one
+// XCD (ids are dealt round-robin to the 8 XCDs): the parts they all read
+// come from that XCD's L2 after the first product, not from HBM R times.
 template <bool ROWC>
 __global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
 {
     __shared__ __attribute__((aligned(16))) double T[SROWS][CHUNK + 1];
-    const int r = blockIdx.z;
-    const int row0 = blockIdx.x * SROWS;
-    const int k0 = blockIdx.y * CHUNK;
+    const int xcd = (int)blockIdx.x & 7, t8 = (int)blockIdx.x >> 3;
+    const int r = t8 % a.R;
+    const int q = (t8 / a.R) * 8 + xcd;             // (row block, chunk) index
+    if (q >= a.nrc) return;
+    const int nrb = a.Lp / SROWS;
+    const int rb = q % nrb, ch = q / nrb;
+    const int row0 = rb * SROWS;
+    const int k0 = ch * CHUNK;
     const int t = threadIdx.x;
     const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
-    // 1. weighted sum of the parts, zero outside their extents
-#pragma unroll 4
-    for (int i = 0; i < SROWS * CHUNK / 256; i++) {
-        const int idx = t + 256 * i;
-        const int row = ROWC ? (idx % SROWS) : (idx / CHUNK);
-        const int kk = ROWC ? (idx / SROWS) : (idx % CHUNK);
-        const int gi = row0 + row, gk = k0 + kk;
-        double v = 0.0;
-        for (int e = e0; e < e1; e++) {
-            const fmm_entry_t en = a.ent[e];
-            const int er = a.swap ? en.cols : en.rows;
-            const int ek = a.swap ? en.rows : en.cols;
-            if (gi < er && gk < ek) v = fma(en.coef, a.X[en.off + (long)gi * a.sr + (long)gk * a.sk], v);
+    // 1. weighted sum of the parts, zero outside their extents: one part
+    //    at a time, the thread's elements of it loaded together
+    constexpr int PER = SROWS * CHUNK / 256;
+    double v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) v[i] = 0.0;
+    for (int e = e0; e < e1; e++) {
+        const fmm_entry_t en = a.ent[e];
+        const int er = a.swap ? en.cols : en.rows;
+        const int ek = a.swap ? en.rows : en.cols;
+        if (er <= 0 || ek <= 0) continue;             // an empty part (dimension below the split)
+        const double* __restrict__ Xe = a.X + en.off;
+        // every load in bounds (clamped into the part) and unconditional, so
+        // they are in flight together; elements outside the part are not added
+        double x[PER];
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int idx = t + 256 * i;
+            const int gi = row0 + (ROWC ? (idx % SROWS) : (idx / CHUNK));
+            const int gk = k0 + (ROWC ? (idx / SROWS) : (idx % CHUNK));
+            x[i] = Xe[(long)min(gi, er - 1) * a.sr + (long)min(gk, ek - 1) * a.sk];
         }
-        T[row][kk] = v;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int idx = t + 256 * i;
+            const int gi = row0 + (ROWC ? (idx % SROWS) : (idx / CHUNK));
+            const int gk = k0 + (ROWC ? (idx / SROWS) : (idx % CHUNK));
+            if (gi < er && gk < ek) v[i] = fma(en.coef, x[i], v[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int idx = t + 256 * i;
+        T[ROWC ? (idx % SROWS) : (idx / CHUNK)][ROWC ? (idx / SROWS) : (idx % CHUNK)] = v[i];
     }
     __syncthreads();
     // 2. per row: scale from the chunk row max, then the three digits
@@ -146,7 +175,7 @@ __global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
         O[(1 * SROWS + row) * CHUNK + kk] = p1;
         O[(2 * SROWS + row) * CHUNK + kk] = p2;
     }
-    if (s == 0) a.E[((long)r * (a.ldk / CHUNK) + blockIdx.y) * a.Lp + row0 + row] = bad ? 0 : -esh;
+    if (s == 0) a.E[((long)r * (a.ldk / CHUNK) + ch) * a.Lp + row0 + row] = bad ? 0 : -esh;
     __syncthreads();
     // 3. write out: per (plane, k step) SROWS rows of 64 B, contiguous
     const long plane = (long)a.ldk * a.Lp;
@@ -472,8 +501,15 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const bool timed = timing_active();
     if (timed) timing_record(&e0, st);
     const dim3 blk(256);
-    const dim3 gA((unsigned)(LpA / bf3::SROWS), (unsigned)nch, (unsigned)R);
-    const dim3 gB((unsigned)(LpB / bf3::SROWS), (unsigned)nch, (unsigned)R);
+    if ((double)round_up(LpA / bf3::SROWS * nch, 8) * R > INT32_MAX ||
+        (double)round_up(LpB / bf3::SROWS * nch, 8) * R > INT32_MAX)
+        return FMMHIP_ERR_UNSUPPORTED;
+    sa.R = sb.R = R;
+    sa.nrc = (int)(LpA / bf3::SROWS * nch);
+    sb.nrc = (int)(LpB / bf3::SROWS * nch);
+    // grid: (row block, chunk) pairs rounded up to 8 (one run per XCD) x products
+    const dim3 gA((unsigned)(round_up(sa.nrc, 8) * R));
+    const dim3 gB((unsigned)(round_up(sb.nrc, 8) * R));
     if (rsA == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gA, blk, 0, st, sa);
     else hipLaunchKernelGGL(bf3::split_kernel<false>, gA, blk, 0, st, sa);
     if (csB == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gB, blk, 0, st, sb);
