@@ -105,17 +105,37 @@ __global__ __launch_bounds__(TB) void presum_parts_kernel(const T* __restrict__ 
                                                           int swap, int Lc, int Ls, int ldo, T* __restrict__ S)
 {
     __shared__ double xs[NPMAX][TB];
+    __shared__ long p_off[NPMAX];                // the parts' offsets and extents, read once
+    __shared__ int p_c[NPMAX], p_s[NPMAX];
     const int t = threadIdx.x;
+    for (int p = t; p < np; p += TB) {
+        const fmm_entry_t pe = parts[p];
+        p_off[p] = pe.off;
+        p_c[p] = swap ? pe.cols : pe.rows;
+        p_s[p] = swap ? pe.rows : pe.cols;
+    }
+    __syncthreads();
     const long nbc = (Lc + TB - 1) / TB;
     const long plane = (long)ldo * Ls;
     for (long blk = blockIdx.x; blk < nbc * Ls; blk += gridDim.x) {
         const int s = (int)(blk / nbc);
         const int c = (int)(blk - (long)s * nbc) * TB + t;
-        for (int p = 0; p < np; p++) {
-            const fmm_entry_t pe = parts[p];
-            const int ec = swap ? pe.cols : pe.rows;
-            const int es = swap ? pe.rows : pe.cols;
-            xs[p][t] = (c < ec && s < es) ? (double)X[pe.off + c + (long)s * ld] : 0.0;
+        // 16 parts at a time, their loads in flight together: every address
+        // is clamped into its part (or to X[0] for an empty part), and
+        // elements outside the part are replaced by zero afterwards
+        for (int p0 = 0; p0 < np; p0 += 16) {
+            double xv[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int p = min(p0 + j, np - 1);
+                const int ec = p_c[p], es = p_s[p];
+                const long at = ((ec > 0) & (es > 0)) ? p_off[p] + min(c, ec - 1) + (long)min(s, es - 1) * ld : 0;
+                const double x = (double)X[at];
+                xv[j] = ((c < ec) & (s < es)) ? x : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (p0 + j < np) xs[p0 + j][t] = xv[j];
         }
         if (c < Lc) {
             for (int r = 0; r < R; r++) {
