@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <mutex>
+#include <thread>
 #include <string.h>
 #include <stdio.h>
 
@@ -97,10 +98,118 @@ extern "C" int fmmhip_free(void* p) { INIT_OR_RETURN(); HIP_OK(hipFree(p)); retu
 // stream), and a pageable hipMemcpy may return before its DMA lands, so the
 // copies below synchronise the device on both sides: on return the data is
 // in place and visible to every stream.
+// Large pageable copies go through pinned bounce buffers, in parallel:
+// kStageThreads host threads each own a contiguous slice of the copy, two
+// pinned chunks and a stream, and overlap their host memcpy of one chunk
+// with the DMA of the other.  A single pageable hipMemcpy runs one staging
+// pipeline (about 24 GB/s measured); the copies of a call are what the
+// drop-in on host objects waits for.
+namespace {
+
+constexpr int kStageThreads = 8;
+constexpr size_t kStageChunk = (size_t)16 << 20;
+constexpr size_t kParallelMin = (size_t)64 << 20;
+
+struct StageWorker {
+    hipStream_t st = nullptr;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+std::mutex g_sw_mu;                          // one parallel copy at a time
+StageWorker g_sw[kStageThreads];
+int g_sw_state = 0;                          // 0 not tried, 1 ready, -1 unavailable
+
+int sw_ready()
+{
+    if (g_sw_state) return g_sw_state > 0;
+    g_sw_state = -1;
+    for (StageWorker& w : g_sw) {
+        if (hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking) != hipSuccess) return 0;
+        for (int i = 0; i < 2; i++) {
+            if (hipHostMalloc(&w.buf[i], kStageChunk, hipHostMallocDefault) != hipSuccess) return 0;
+            if (hipEventCreateWithFlags(&w.ev[i], hipEventDisableTiming) != hipSuccess) return 0;
+        }
+    }
+    g_sw_state = 1;
+    return 1;
+}
+
+// host -> device: chunk c is copied into pinned buffer c&1 once that buffer's
+// previous DMA has completed, then its DMA is queued.
+bool slice_h2d(StageWorker& w, char* d, const char* s, size_t bytes)
+{
+    int i = 0;
+    for (size_t off = 0; off < bytes; off += kStageChunk, i ^= 1) {
+        const size_t n = std::min(kStageChunk, bytes - off);
+        if (hipEventSynchronize(w.ev[i]) != hipSuccess) return false;
+        memcpy(w.buf[i], s + off, n);
+        if (hipMemcpyAsync(d + off, w.buf[i], n, hipMemcpyHostToDevice, w.st) != hipSuccess) return false;
+        if (hipEventRecord(w.ev[i], w.st) != hipSuccess) return false;
+    }
+    return hipStreamSynchronize(w.st) == hipSuccess;
+}
+
+// device -> host: the DMA of chunk c+1 is queued before chunk c is copied out.
+bool slice_d2h(StageWorker& w, char* d, const char* s, size_t bytes)
+{
+    const size_t nc = (bytes + kStageChunk - 1) / kStageChunk;
+    auto issue = [&](size_t c) {
+        const size_t off = c * kStageChunk, n = std::min(kStageChunk, bytes - off);
+        return hipMemcpyAsync(w.buf[c & 1], s + off, n, hipMemcpyDeviceToHost, w.st) == hipSuccess &&
+               hipEventRecord(w.ev[c & 1], w.st) == hipSuccess;
+    };
+    if (nc && !issue(0)) return false;
+    for (size_t c = 0; c < nc; c++) {
+        if (c + 1 < nc && !issue(c + 1)) return false;
+        if (hipEventSynchronize(w.ev[c & 1]) != hipSuccess) return false;
+        const size_t off = c * kStageChunk;
+        memcpy(d + off, w.buf[c & 1], std::min(kStageChunk, bytes - off));
+    }
+    return hipStreamSynchronize(w.st) == hipSuccess;
+}
+
+// Splits the copy into kStageThreads page-aligned slices; false if the
+// pinned path is unavailable (the caller then uses a plain hipMemcpy).
+bool parallel_copy(void* d, const void* s, size_t b, bool h2d, int* rc)
+{
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    if (!sw_ready()) return false;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const size_t per = ((b + kStageThreads - 1) / kStageThreads + 4095) & ~(size_t)4095;
+    bool ok[kStageThreads];
+    std::thread th[kStageThreads];
+    auto work = [&](int t) {
+        const size_t off = std::min(b, per * t), n = std::min(per, b - off);
+        if (hipSetDevice(dev) != hipSuccess) { ok[t] = false; return; }
+        ok[t] = h2d ? slice_h2d(g_sw[t], (char*)d + off, (const char*)s + off, n)
+                    : slice_d2h(g_sw[t], (char*)d + off, (const char*)s + off, n);
+    };
+    for (int t = 0; t < kStageThreads; t++) {
+        ok[t] = true;
+        try {
+            th[t] = std::thread(work, t);
+        } catch (...) {
+            work(t);                 // no thread available: this slice on the caller's thread
+        }
+    }
+    for (std::thread& x : th)
+        if (x.joinable()) x.join();
+    *rc = FMMHIP_SUCCESS;
+    for (int t = 0; t < kStageThreads; t++)
+        if (!ok[t]) *rc = FMMHIP_ERR_HIP;
+    return true;
+}
+
+}  // namespace
+
 extern "C" int fmmhip_memcpy_h2d(void* d, const void* s, size_t b)
 {
     INIT_OR_RETURN();
     HIP_OK(hipDeviceSynchronize());
+    int rc = 0;
+    if (b >= kParallelMin && parallel_copy(d, s, b, true, &rc)) return rc;
     HIP_OK(hipMemcpy(d, s, b, hipMemcpyHostToDevice));
     HIP_OK(hipDeviceSynchronize());
     return 0;
@@ -109,6 +218,8 @@ extern "C" int fmmhip_memcpy_d2h(void* d, const void* s, size_t b)
 {
     INIT_OR_RETURN();
     HIP_OK(hipDeviceSynchronize());
+    int rc = 0;
+    if (b >= kParallelMin && parallel_copy(d, s, b, false, &rc)) return rc;
     HIP_OK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost));
     HIP_OK(hipDeviceSynchronize());
     return 0;
