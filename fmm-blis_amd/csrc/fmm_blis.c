@@ -322,9 +322,13 @@ static void fmm_driver(const char* who, obj_t* alpha, obj_t* A, obj_t* B, obj_t*
             pb = g_stage[1].p;
         }
         if (dev_c != 1) {
-            size_t bytes = vc.esz * (size_t)span_elems(vc.m, vc.n, vc.rs, vc.cs);
+            const long span = span_elems(vc.m, vc.n, vc.rs, vc.cs);
+            size_t bytes = vc.esz * (size_t)span;
             if ((rc = stage_reserve(&g_stage[2], bytes))) goto out;
-            if ((rc = fmmhip_memcpy_h2d(g_stage[2].p, vc.p, bytes))) goto out;
+            /* beta == 0 never reads C: a C without gaps between its columns
+             * (or rows) need not be uploaded; with gaps it is, so that the
+             * copy back returns the gap elements unchanged */
+            if (!(b_val == 0.0 && span == m * n) && (rc = fmmhip_memcpy_h2d(g_stage[2].p, vc.p, bytes))) goto out;
             pc = g_stage[2].p;
         }
     }

@@ -237,6 +237,28 @@ def test_dropin_row_stored_host_and_levels():
     assert np.array_equal(Cr, want)
 
 
+def test_dropin_host_beta_zero_skips_c_and_keeps_gaps():
+    """beta = 0 on host objects: C is never read (NaN in C does not reach the
+    result), and a C with gaps between its columns (ld > m) keeps the gap
+    elements unchanged through the staging round trip."""
+    m, n, k = 150, 130, 140
+    A, B = O.integers(m, k, 13, 0, 3), O.integers(k, n, 13, 1, 3)
+    want = A @ B
+    one, s1 = F.obj_scalar(1.0)
+    zero, s0 = F.obj_scalar(0.0)
+    Cn = np.full((m, n), np.nan, order="F")
+    F.lib().bli_strassen_ab_ex(one, F.obj_attach(A, m, k, 1, m), F.obj_attach(B, k, n, 1, k), zero,
+                               F.obj_attach(Cn, m, n, 1, m), fmm("222"))
+    assert np.array_equal(Cn, want)
+    ldc = m + 5
+    big = np.full((ldc, n), 7.0, order="F")
+    big[:m, :] = np.nan
+    F.lib().bli_strassen_ab_ex(one, F.obj_attach(A, m, k, 1, m), F.obj_attach(B, k, n, 1, k), zero,
+                               F.obj_attach(big, m, n, 1, ldc), fmm("222"))
+    assert np.array_equal(big[:m, :], want)
+    assert np.all(big[m:, :] == 7.0)
+
+
 def test_classical_object_entry_float():
     m, n, k = 64, 80, 96
     A, B, C = O.uniform_f32(m, k, 1, 0), O.uniform_f32(k, n, 1, 1), O.uniform_f32(m, n, 1, 2)
