@@ -423,6 +423,7 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=0.0, help="serve: HBM budget (0: 85%% of free memory)")
     ap.add_argument("--max-dim", type=int, default=16384, help="serve: largest m, n, k")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tuner-table", default="", help="batch/serve: load this tuner table (fmmhip_tuner_save format)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU sharding is a "
@@ -431,6 +432,8 @@ def main():
 
     import fmm_blis_amd as F
     F.check(F.lib().fmmhip_init(), "init")
+    if args.tuner_table:
+        F.tuner_load(args.tuner_table)
     if args.form == "batch":
         return run_batch(args, F)
     if args.form == "serve":

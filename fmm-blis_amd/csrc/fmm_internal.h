@@ -93,9 +93,12 @@ typedef struct fmmhip_presum_seg_s {
 } fmmhip_presum_seg_t;
 
 /* Batched summation: segment s is processed by blocks [boff[s], boff[s+1]),
- * boff has nseg+1 entries, nblocks = boff[nseg] (all device memory). */
+ * boff has nseg+1 entries, nblocks = boff[nseg] (all device memory).
+ * parts: every segment has its part table with np <= FMMHIP_BATCH_PARTS_MAX
+ * (part-staged form, same sums bit for bit); else the entry-driven form. */
+#define FMMHIP_BATCH_PARTS_MAX 16
 int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff, int nblocks,
-                        void* stream);
+                        int parts, void* stream);
 
 /* Summation stage of the pre-summed pack mode (fmmhip_presum.hip). */
 int fmmhip_presum_d(const double* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,

@@ -45,9 +45,9 @@ struct TunerParams {
 std::mutex g_tuner_mu;
 TunerParams g_tp = {
     // defaults: fmmhip_tuner_calibrate on an MI355X (FP32 row, then FP64)
-    {{0.62, 0.66, 0.64}, {0.79, 0.83, 0.81}},
+    {{0.71, 0.69, 0.66}, {0.88, 0.87, 0.83}},
     {{4.0, 4.0, 4.0}, {4.0, 4.0, 4.0}},
-    3.2e12, 0.01, 256};
+    3.9e12, 0.01, 256};
 
 const double kPeak[2] = {157.3e12, 78.6e12};   // FP32, FP64 MFMA peaks (FLOP/s)
 
@@ -92,13 +92,14 @@ int choose(int is_double, long m, long n, long k, bool lone, double* best_ms)
         tp = g_tp;
     }
     int best = 0;
-    double bt = 1e300;
+    double bt = 1e300, tmin = 1e300;   // the chosen estimate; the least over every simpler candidate
     for (int c = 0; c < kCands; c++) {
         // a level needs leaves of at least two tiles and k of 8 BK-steps
         const CandShape& s = kShape[c];
         if (c > 0 && (m < 2L * fmmk::BM * s.mt || n < 2L * fmmk::BN * s.nt || k < 8L * fmmk::BK * s.kt)) continue;
         const double t = est_ms(tp, is_double, c, m, n, k, lone);
-        if (c == 0 || t < kFmmMargin * bt) { bt = t; best = c; }
+        if (c == 0 || t < kFmmMargin * tmin) { bt = t; best = c; }
+        tmin = std::min(tmin, t);
     }
     if (best_ms) *best_ms = bt;
     return best;
@@ -139,12 +140,14 @@ extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, lo
     return FMMHIP_SUCCESS;
 }
 
-// Times each candidate at the square size whose leaf fills the device with
-// whole waves of tiles (classical 4096^3, one level 8192^3, two levels
-// 16384^3: 1024 tiles of 128x128 each) with the library's launch timing,
-// and refits its sustained efficiency (at the fixed prologue cost ovh) and
-// the summation bandwidth.  Sizes whose leaf leaves the device partly idle
-// would fold the wave quantisation into the efficiency.
+// Times each candidate at 16384^3 (leaves of 16384, 8192 and 4096: 16384,
+// 4096 and 1024 tiles of 128x128, whole waves of the device's workgroup
+// slots) with the library's launch timing, and refits its sustained
+// efficiency (at the fixed prologue cost ovh) and the summation bandwidth.
+// Large, device-filling launches give the steady-state efficiency that a
+// batch (no tails) and a busy server see; smaller calibration sizes folded
+// the launch tail into the classical figure and made the model favour FMM
+// where it measured slower.
 extern "C" int fmmhip_tuner_calibrate(void* stream)
 {
     int rc = fmmhip_init();
@@ -182,7 +185,7 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
             const int saved_pack = alg->pack, saved_split = alg->split;
             alg->pack = cand > 0 ? 1 : 0;
             alg->split = 1;             // as the batch and the server run them
-            const long nn = 4096L << cand;
+            const long nn = nmax;
             for (int rep = 0; rep < 3 && !rc; rep++) {
                 if (rep == 1) fmmhip_timing_enable(1);
                 if (dbl)
@@ -435,6 +438,7 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     std::vector<int> boff[2];
     bool vec[2] = {true, true};
     std::vector<size_t> scale_only;
+    bool parts[2] = {true, true};     // part-staged summation for every segment of the datatype
     for (int d = 0; d < 2; d++) boff[d].push_back(0);
     for (size_t i = 0; i < np; i++) {
         const Prepared& P = b->prep[i];
@@ -451,10 +455,8 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         }
         if (P.presum) {
             for (const fmmhip_presum_seg_t* sg : {&P.segA, &P.segB}) {
-                const long pairs = (long)((sg->Lc + 1) / 2) * sg->Ls;
-                const int nbl = (int)std::max(1L, (pairs + 256L * 16 - 1) / (256L * 16));
                 segs[d].push_back(*sg);
-                boff[d].push_back(boff[d].back() + nbl);
+                parts[d] = parts[d] && sg->parts && sg->np > 0 && sg->np <= FMMHIP_BATCH_PARTS_MAX;
             }
         }
     }
@@ -467,6 +469,14 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         items[d].swap(sorted);
         b->items[d] = (long)items[d].size();
     }
+    // block ranges of the summation segments: part-staged units are 256
+    // elements along c (32 per block), entry-driven ones pairs (4096 per block)
+    for (int d = 0; d < 2; d++)
+        for (const fmmhip_presum_seg_t& sg : segs[d]) {
+            const long work = parts[d] ? (long)((sg.Lc + 255) / 256) * sg.Ls : (long)((sg.Lc + 1) / 2) * sg.Ls;
+            const long per = parts[d] ? 32 : 4096;
+            boff[d].push_back(boff[d].back() + (int)std::max(1L, (work + per - 1) / per));
+        }
     // 3. one device block: [ka0 | ka1 | items0 | items1 | segs0 | segs1 | boff0 | boff1 | head0 head1]
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) & ~(size_t)255; return o; };
@@ -505,7 +515,7 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     for (int d = 1; d >= 0 && !rc; d--) {
         if (!segs[d].empty())
             rc = fmmhip_presum_batch(d, (const fmmhip_presum_seg_t*)(dv + o_sg[d]), (int)segs[d].size(),
-                                     (const int*)(dv + o_bo[d]), boff[d].back(), b->stream);
+                                     (const int*)(dv + o_bo[d]), boff[d].back(), parts[d] ? 1 : 0, b->stream);
         if (!rc && !items[d].empty()) {
             const int nwg = (int)std::min<long>((long)items[d].size(), 2L * cus);
             int* head = (int*)(dv + o_head) + d;

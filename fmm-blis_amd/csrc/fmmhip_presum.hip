@@ -97,6 +97,56 @@ __global__ __launch_bounds__(256) void presum_batch_kernel(const fmmhip_presum_s
 // the entries' coefficients in CSR order -- the same FMA sequence as
 // presum_kernel, so the sums are bit-identical; HBM reads drop from one per
 // (entry, element) to one per (part, element).
+// The parts' offsets and extents of one operand, into LDS (once per block).
+template <int NPMAX, int TB>
+__device__ __forceinline__ void parts_meta(const fmm_entry_t* __restrict__ parts, int np, int swap, long* p_off,
+                                           int* p_c, int* p_s)
+{
+    for (int p = threadIdx.x; p < np; p += TB) {
+        const fmm_entry_t pe = parts[p];
+        p_off[p] = pe.off;
+        p_c[p] = swap ? pe.cols : pe.rows;
+        p_s[p] = swap ? pe.rows : pe.cols;
+    }
+}
+
+// One unit: element (c, s) of every part into the thread's LDS slots, then
+// every S_r at (c, s).  16 parts at a time, their loads in flight together:
+// every address is clamped into its part (or to X[0] for an empty part), and
+// elements outside the part are replaced by zero afterwards.
+template <typename T, int NPMAX, int TB>
+__device__ __forceinline__ void parts_unit(const T* __restrict__ X, long ld, const int* __restrict__ ptr,
+                                           const fmm_entry_t* __restrict__ ent, int np, int R, int Lc, int ldo,
+                                           long plane, T* __restrict__ S, int s, int c, double (*xs)[TB],
+                                           const long* p_off, const int* p_c, const int* p_s)
+{
+    const int t = threadIdx.x;
+    for (int p0 = 0; p0 < np; p0 += 16) {
+        double xv[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int p = min(p0 + j, np - 1);
+            const int ec = p_c[p], es = p_s[p];
+            const long at = ((ec > 0) & (es > 0)) ? p_off[p] + min(c, ec - 1) + (long)min(s, es - 1) * ld : 0;
+            const double x = (double)X[at];
+            xv[j] = ((c < ec) & (s < es)) ? x : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (p0 + j < np) xs[p0 + j][t] = xv[j];
+    }
+    if (c < Lc) {
+        for (int r = 0; r < R; r++) {
+            double v = 0.0;
+            for (int e = ptr[r]; e < ptr[r + 1]; e++) {
+                const fmm_entry_t en = ent[e];
+                v = fma(en.coef, xs[en.part][t], v);
+            }
+            S[r * plane + c + (long)s * ldo] = (T)v;
+        }
+    }
+}
+
 template <typename T, int NPMAX, int TB>
 __global__ __launch_bounds__(TB) void presum_parts_kernel(const T* __restrict__ X, long ld,
                                                           const int* __restrict__ ptr,
@@ -105,48 +155,46 @@ __global__ __launch_bounds__(TB) void presum_parts_kernel(const T* __restrict__ 
                                                           int swap, int Lc, int Ls, int ldo, T* __restrict__ S)
 {
     __shared__ double xs[NPMAX][TB];
-    __shared__ long p_off[NPMAX];                // the parts' offsets and extents, read once
+    __shared__ long p_off[NPMAX];
     __shared__ int p_c[NPMAX], p_s[NPMAX];
-    const int t = threadIdx.x;
-    for (int p = t; p < np; p += TB) {
-        const fmm_entry_t pe = parts[p];
-        p_off[p] = pe.off;
-        p_c[p] = swap ? pe.cols : pe.rows;
-        p_s[p] = swap ? pe.rows : pe.cols;
-    }
+    parts_meta<NPMAX, TB>(parts, np, swap, p_off, p_c, p_s);
     __syncthreads();
     const long nbc = (Lc + TB - 1) / TB;
     const long plane = (long)ldo * Ls;
     for (long blk = blockIdx.x; blk < nbc * Ls; blk += gridDim.x) {
         const int s = (int)(blk / nbc);
-        const int c = (int)(blk - (long)s * nbc) * TB + t;
-        // 16 parts at a time, their loads in flight together: every address
-        // is clamped into its part (or to X[0] for an empty part), and
-        // elements outside the part are replaced by zero afterwards
-        for (int p0 = 0; p0 < np; p0 += 16) {
-            double xv[16];
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int p = min(p0 + j, np - 1);
-                const int ec = p_c[p], es = p_s[p];
-                const long at = ((ec > 0) & (es > 0)) ? p_off[p] + min(c, ec - 1) + (long)min(s, es - 1) * ld : 0;
-                const double x = (double)X[at];
-                xv[j] = ((c < ec) & (s < es)) ? x : 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-                if (p0 + j < np) xs[p0 + j][t] = xv[j];
-        }
-        if (c < Lc) {
-            for (int r = 0; r < R; r++) {
-                double v = 0.0;
-                for (int e = ptr[r]; e < ptr[r + 1]; e++) {
-                    const fmm_entry_t en = ent[e];
-                    v = fma(en.coef, xs[en.part][t], v);
-                }
-                S[r * plane + c + (long)s * ldo] = (T)v;
-            }
-        }
+        const int c = (int)(blk - (long)s * nbc) * TB + threadIdx.x;
+        parts_unit<T, NPMAX, TB>(X, ld, ptr, ent, np, R, Lc, ldo, plane, S, s, c, xs, p_off, p_c, p_s);
+    }
+}
+
+// Batched part-staged form (segments with at most NPMAX parts): blocks
+// [boff[g], boff[g+1]) take the units (TB elements along c of one s) of
+// segment g in a grid stride.
+template <typename T, int NPMAX, int TB>
+__global__ __launch_bounds__(TB) void presum_parts_batch_kernel(const fmmhip_presum_seg_t* __restrict__ segs,
+                                                                int nseg, const int* __restrict__ boff)
+{
+    int lo = 0, hi = nseg - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boff[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const fmmhip_presum_seg_t sg = segs[lo];
+    __shared__ double xs[NPMAX][TB];
+    __shared__ long p_off[NPMAX];
+    __shared__ int p_c[NPMAX], p_s[NPMAX];
+    parts_meta<NPMAX, TB>(sg.parts, sg.np, sg.swap, p_off, p_c, p_s);
+    __syncthreads();
+    const long nbc = (sg.Lc + TB - 1) / TB;
+    const long plane = (long)sg.ldo * sg.Ls;
+    const int nbl = boff[lo + 1] - boff[lo];
+    for (long u = b - boff[lo]; u < nbc * sg.Ls; u += nbl) {
+        const int s = (int)(u / nbc);
+        const int c = (int)(u - (long)s * nbc) * TB + threadIdx.x;
+        parts_unit<T, NPMAX, TB>((const T*)sg.X, sg.ld, sg.ptr, sg.ent, sg.np, sg.R, sg.Lc, sg.ldo, plane, (T*)sg.S, s,
+                                 c, xs, p_off, p_c, p_s);
     }
 }
 
@@ -183,9 +231,18 @@ extern "C" int fmmhip_presum_seg(int is_double, const fmmhip_presum_seg_t* seg, 
 }
 
 extern "C" int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff,
-                                   int nblocks, void* stream)
+                                   int nblocks, int parts, void* stream)
 {
     if (nseg <= 0 || nblocks <= 0) return FMMHIP_SUCCESS;
+    if (parts) {
+        if (is_double)
+            hipLaunchKernelGGL((presum_parts_batch_kernel<double, FMMHIP_BATCH_PARTS_MAX, 256>), dim3(nblocks), dim3(256), 0,
+                               (hipStream_t)stream, segs, nseg, boff);
+        else
+            hipLaunchKernelGGL((presum_parts_batch_kernel<float, FMMHIP_BATCH_PARTS_MAX, 256>), dim3(nblocks), dim3(256), 0,
+                               (hipStream_t)stream, segs, nseg, boff);
+        return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+    }
     if (is_double)
         hipLaunchKernelGGL(presum_batch_kernel<double>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, segs, nseg,
                            boff);
