@@ -487,6 +487,7 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const long tmp_elems = groups > 1 ? (((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L) : 0;
     const size_t oT = oEB + al(expB);
     const size_t bytes = oT + (size_t)groups * tmp_elems * 8;
+    std::unique_lock<std::mutex> lk = enqueue_lock();   // held until the last launch below
     char* ws = (char*)workspace_for(st, bytes);
     if (!ws) return FMMHIP_ERR_NOMEM;
 

@@ -526,6 +526,9 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
     hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
     Prepared P;
     if (!alg) return FMMHIP_ERR_INVALID;
+    // from the workspace lookup to the last launch: a call from another host
+    // thread on the same stream cannot regrow the workspace in between
+    std::unique_lock<std::mutex> lk = enqueue_lock();
     const int pack = alg->pack;
     rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, pack, alg->split, st,
                       &P);
@@ -555,6 +558,9 @@ int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int*
 }
 
 void* workspace_for(hipStream_t st, size_t bytes) { return workspace_get(st, bytes); }
+
+std::mutex g_enqueue_mu;
+std::unique_lock<std::mutex> enqueue_lock() { return std::unique_lock<std::mutex>(g_enqueue_mu); }
 
 namespace {
 size_t list_bytes(int R, long nent)

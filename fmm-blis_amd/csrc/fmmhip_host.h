@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "fmmhip_kernel.h"
 
 enum class Kind { D64, D_F32LEAF, S32 };
@@ -76,6 +78,10 @@ int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int*
                const fmm_entry_t** dent);
 // The stream's grow-only workspace of at least `bytes` (nullptr: no memory).
 void* workspace_for(hipStream_t st, size_t bytes);
+// Held by a call from its workspace lookup to its last launch (host-side
+// enqueue only), so that concurrent calls on one stream keep their
+// workspace while they enqueue.
+std::unique_lock<std::mutex> enqueue_lock();
 // Launch timing hooks (fmmhip_timing_*): whether timing is on, an event
 // recorded on `st`, and one timed call (summation/split from e0 to e1,
 // MFMA kernel from e1 to e2).

@@ -373,3 +373,46 @@ def test_large_host_copies_round_trip():
     F.dgemm(m, 1, m, 1.0, dA, dx, 0.0, dy)
     F.sync()
     assert np.array_equal(dy.to_numpy(), A @ x)
+
+
+def test_concurrent_host_threads_on_one_stream():
+    """Four host threads issue pre-summed and product-split FMM calls of
+    growing sizes on the library's default stream at once (each call may
+    regrow the stream's workspace): every result equals the oracle's FMM
+    bit for bit on integer inputs."""
+    import threading
+    og = O.parse(alg_path("222"))
+    og2 = O.compose(og, og)
+    cases = []
+    for t in range(4):
+        for j, (m, n, k) in enumerate([(130, 120, 110), (260, 250, 240), (520, 500, 480)]):
+            A, B, C = (O.integers(m, k, 20 + t, 3 * j, 3), O.integers(k, n, 20 + t, 3 * j + 1, 3),
+                       O.integers(m, n, 20 + t, 3 * j + 2, 3))
+            cases.append((t, A, B, C, dev(A), dev(B), dev(C)))
+    errors = []
+
+    def worker(t):
+        try:
+            a1 = F.Alg([F.strassen()])
+            a2 = F.Alg([F.strassen()] * 2)
+            a2.set_split(4 if t % 2 else -1)
+            for (tt, A, B, C, dA, dB, dC) in cases:
+                if tt != t:
+                    continue
+                m, k = A.shape
+                n = B.shape[1]
+                F.dgemm_fmm(a2 if m > 200 else a1, m, n, k, 1.0, dA, dB, 1.0, dC)
+        except Exception as e:          # reported below, on the main thread
+            errors.append(e)
+
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    F.sync()
+    assert not errors, errors
+    for (t, A, B, C, dA, dB, dC) in cases:
+        og_used = og2 if A.shape[0] > 200 else og
+        want = O.fmm_dgemm(og_used, 1.0, A, B, 1.0, C.copy(order="F"))
+        assert np.array_equal(dC.to_numpy(), want), (t, A.shape)
