@@ -482,8 +482,9 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const size_t oPB = al(planesA), oEA = oPB + al(planesB), oEB = oEA + al(expA);
     const long tiles = (LpA / Cfg3::BM) * (LpB / Cfg3::BN);
     // product split (bf16 MFMA rate: 8 or 6 plane products per leaf product)
+    const double c_span = (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0;   // the temporaries' extent
     const int groups = split_groups(alg->split, R, tiles, 2.0 * terms * R * (double)g.lm * (double)g.lk * (double)g.ln,
-                                    8.0 * (double)m * (double)n, false, 2.5e15, 1);   // one workgroup per CU (LDS)
+                                    8.0 * c_span, false, 2.5e15, 1);   // one workgroup per CU (LDS)
     const long tmp_elems = groups > 1 ? (((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L) : 0;
     const size_t oT = oEB + al(expB);
     const size_t bytes = oT + (size_t)groups * tmp_elems * 8;

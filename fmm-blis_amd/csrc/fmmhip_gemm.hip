@@ -270,12 +270,12 @@ constexpr double kWsFractionOfFree = 0.5;
 
 // Product groups per tile for a grid of `tiles` workgroups (peak: the
 // leaf's MFMA peak in FLOP/s, 0 = FP64 or FP32 by `fp64`; wg_per_cu: the
-// kernel's resident workgroups per CU).  Time model per choice G: the leaf
-// flops at a sustained MFMA rate scaled by the fraction of the last wave
-// of workgroup slots that is filled,
-// plus, for G > 1, the temporaries' traffic at HBM speed (G copies of C
-// zeroed, read back by the reduction, C read and written once).  G is a
-// power of two <= min(8, R); ties go to fewer groups.
+// kernel's resident workgroups per CU; c_bytes: C's extent in bytes, gaps
+// included).  Time model per choice G: the leaf flops at a sustained MFMA
+// rate scaled by the fraction of the last wave of workgroup slots that is
+// filled, plus, for G > 1, the temporaries' traffic at HBM speed (G copies
+// of C zeroed, read back by the reduction, C read and written once).  G is
+// a power of two <= min(8, R); ties go to fewer groups.
 int split_groups(int requested, int R, long tiles, double leaf_flops, double c_bytes, bool fp64, double peak,
                  int wg_per_cu)
 {
@@ -426,8 +426,10 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     P->c_col = rsC == 1;
     const long tiles = (long)ka.tiles_m * ka.tiles_n;
     // product split: group g of every tile scatters into temporary g
+    // (the temporaries span C's whole extent, gaps between columns included)
+    const double c_span = (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0;
     ka.groups = split_groups(split, alg->R, tiles, 2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln,
-                             (double)esz * (double)m * (double)n, kind == Kind::D64, 0.0, 2);
+                             (double)esz * c_span, kind == Kind::D64, 0.0, 2);
     ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
     ka.tmp = nullptr;
     ka.tmp_elems = 0;
