@@ -585,6 +585,12 @@ def main():
                      "kernel_ms": round(kern_ms, 3),
                      "mfma_flops_per_launch": mfma_fl, "effective_over_peak": round(eff / 1e3 / peak, 4)},
     }
+    if args.leaf == "f64" and not args.profile:
+        # the device's own FP64 MFMA rate and copy bandwidth, beside the published peaks
+        t_meas, gbs = F.measure_peaks()
+        out["roofline"]["measured_peak"] = round(t_meas, 2)
+        out["roofline"]["frac_of_measured"] = round(achieved / t_meas, 4)
+        out["roofline"]["measured_copy_gbs"] = round(gbs, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

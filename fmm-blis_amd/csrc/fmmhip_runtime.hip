@@ -249,6 +249,73 @@ extern "C" int fmmhip_copy_kernel(void* d, const void* s, size_t b, void* stream
     return 0;
 }
 
+// ---- measured peaks -----------------------------------------------------------
+
+typedef double peak_d4 __attribute__((ext_vector_type(4)));
+
+// Eight independent accumulator chains of v_mfma_f64_16x16x4_f64 per wave,
+// no memory traffic: the FP64 matrix-core rate the device sustains (four
+// waves per SIMD, accumulators kept in the architectural registers).  The
+// sum is stored (one double per block) so that the work is not removed.
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(4, 4)))
+void mfma_f64_peak_kernel(double* __restrict__ out, int iters)
+{
+    peak_d4 acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = (peak_d4){0.0, 0.0, 0.0, 0.0};
+    const double a = 1.0 + 1e-9 * threadIdx.x, b = 1.0 - 1e-9 * threadIdx.x;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) t += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+extern "C" int fmmhip_measure_peaks(double* fp64_mfma_tflops, double* copy_gbs)
+{
+    INIT_OR_RETURN();
+    hipStream_t st = g_stream;
+    const int blocks = g_cus * 4;               // 4 waves per block: 4 waves per SIMD
+    const int iters = 40000;
+    double* out = nullptr;
+    void *x = nullptr, *y = nullptr;
+    const size_t cb = (size_t)4 << 30;          // 4 GiB copied
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FMMHIP_ERR_HIP;
+    float ms = 0.f, best_mfma = 1e30f, best_copy = 1e30f;
+    if (hipMalloc(&out, sizeof(double) * blocks) != hipSuccess) return FMMHIP_ERR_NOMEM;
+    if (hipMalloc(&x, cb) != hipSuccess || hipMalloc(&y, cb) != hipSuccess) { rc = FMMHIP_ERR_NOMEM; goto done; }
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto done;
+    if (hipMemsetAsync(x, 0, cb, st) != hipSuccess) goto done;
+    for (int rep = 0; rep < 4; rep++) {         // the first of each is a warm-up
+        if (hipEventRecord(e0, st) != hipSuccess) goto done;
+        hipLaunchKernelGGL(mfma_f64_peak_kernel, dim3(blocks), dim3(256), 0, st, out, iters);
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) goto done;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) goto done;
+        if (rep) best_mfma = std::min(best_mfma, ms);
+        if (hipEventRecord(e0, st) != hipSuccess) goto done;
+        if (fmmhip_copy_kernel(y, x, cb, st)) goto done;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) goto done;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) goto done;
+        if (rep) best_copy = std::min(best_copy, ms);
+    }
+    if (fp64_mfma_tflops) *fp64_mfma_tflops = (double)blocks * 4 * iters * 8 * 2048.0 / (best_mfma * 1e-3) / 1e12;
+    if (copy_gbs) *copy_gbs = 2.0 * (double)cb / (best_copy * 1e-3) / 1e9;
+    rc = FMMHIP_SUCCESS;
+done:
+    (void)hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    if (out) (void)hipFree(out);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    return rc;
+}
+
 extern "C" int fmmhip_memset(void* p, int v, size_t b)
 {
     INIT_OR_RETURN();

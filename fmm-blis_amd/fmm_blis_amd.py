@@ -123,6 +123,7 @@ _SIGS = {
     "fmmhip_timing_read": (_I, [ctypes.POINTER(_L)] + [ctypes.POINTER(_D)] * 3),
     "fmmhip_memcpy_d2d": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "fmmhip_copy_kernel": (_I, [_P, _P, ctypes.c_size_t, _P]),
+    "fmmhip_measure_peaks": (_I, [ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
@@ -519,6 +520,13 @@ def fro(X):
 
 def sync():
     check(lib().fmmhip_device_sync(), "device_sync")
+
+
+def measure_peaks():
+    """(FP64 MFMA TFLOP/s, copy GB/s) measured on the device."""
+    t, g = _D(), _D()
+    check(lib().fmmhip_measure_peaks(ctypes.byref(t), ctypes.byref(g)), "measure_peaks")
+    return t.value, g.value
 
 
 def timing_enable(on=True):

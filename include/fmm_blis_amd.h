@@ -364,6 +364,11 @@ int fmmhip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
  * pointers multiples of 16): a workload of known size for calibrating the
  * HBM byte counters of a profiler. */
 int fmmhip_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
+/* Measured peaks (no counterpart in the reference; for rooflines): the FP64
+ * MFMA rate of back-to-back independent v_mfma_f64_16x16x4_f64 chains on
+ * every SIMD (TFLOP/s), and the read+write rate of a 4 GiB copy (GB/s).
+ * Takes about a second and 8 GiB of device memory. */
+int fmmhip_measure_peaks(double* fp64_mfma_tflops, double* copy_gbs);
 int fmmhip_stream_create(void** stream);
 int fmmhip_stream_destroy(void* stream);
 int fmmhip_stream_sync(void* stream);
