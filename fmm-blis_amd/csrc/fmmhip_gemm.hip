@@ -268,21 +268,23 @@ constexpr double kWsFractionOfFree = 0.5;
 
 }  // namespace
 
-// Product groups per tile for a grid of `tiles` workgroups (peak: the
+// Groups per tile (split of the products, or of the k-steps of a
+// classical call) for a grid of `tiles` workgroups (max_groups: the products,
+// or the k-steps / 8; peak: the
 // leaf's MFMA peak in FLOP/s, 0 = FP64 or FP32 by `fp64`; wg_per_cu: the
 // kernel's resident workgroups per CU; c_bytes: C's extent in bytes, gaps
 // included).  Time model per choice G: the leaf flops at a sustained MFMA
 // rate scaled by the fraction of the last wave of workgroup slots that is
 // filled, plus, for G > 1, the temporaries' traffic at HBM speed (G copies
 // of C zeroed, read back by the reduction, C read and written once).  G is
-// a power of two <= min(8, R); ties go to fewer groups.
-int split_groups(int requested, int R, long tiles, double leaf_flops, double c_bytes, bool fp64, double peak,
-                 int wg_per_cu)
+// a power of two <= min(8, max_groups); ties go to fewer groups.
+int split_groups(int requested, int max_groups, long tiles, double leaf_flops, double c_bytes, bool fp64,
+                 double peak, int wg_per_cu)
 {
-    if (R < 2 || requested == 1) return 1;
+    if (max_groups < 2 || requested == 1) return 1;
     if (requested > 1) {
         int g = std::min(requested, 8);
-        while (g > R) g /= 2;
+        while (g > max_groups) g /= 2;
         return g;
     }
     int cus = 256;
@@ -292,7 +294,7 @@ int split_groups(int requested, int R, long tiles, double leaf_flops, double c_b
     const double hbm = 4.0e12;                                // achieved bytes/s of the fill and reduction
     int best = 1;
     double best_t = 0.0;
-    for (int g = 1; g <= 8 && g <= R; g *= 2) {
+    for (int g = 1; g <= 8 && g <= max_groups; g *= 2) {
         const double items = (double)tiles * g;
         const double fill = items / (std::ceil(items / slots) * slots);
         double t = leaf_flops / (rate * fill);
@@ -425,11 +427,22 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     P->b_mn = b_mn;
     P->c_col = rsC == 1;
     const long tiles = (long)ka.tiles_m * ka.tiles_n;
-    // product split: group g of every tile scatters into temporary g
-    // (the temporaries span C's whole extent, gaps between columns included)
+    // split: group g of every tile scatters into temporary g (the
+    // temporaries span C's whole extent, gaps between columns included);
+    // the groups take contiguous product ranges, or for the classical
+    // algorithm contiguous k-step ranges of at least 8 steps
     const double c_span = (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0;
-    ka.groups = split_groups(split, alg->R, tiles, 2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln,
-                             (double)esz * c_span, kind == Kind::D64, 0.0, 2);
+    // classical, column-major A, B, C, and at most half the CUs with a tile:
+    // a lone workgroup per CU runs at about 0.9 of two (measured), so a grid
+    // of one tile per CU gains too little to pay for the temporaries
+    int cus = 256;
+    fmmhip_device_cu_count(&cus);
+    const bool kspl = alg->R == 1 && a_mn && !b_mn && rsC == 1 && 2 * tiles <= cus;
+    const long ksteps = (g.lk + fmmk::BK - 1) / fmmk::BK;
+    ka.groups = split_groups(split, alg->R == 1 ? (kspl ? (int)std::min(ksteps / 8, 8L) : 1) : alg->R, tiles,
+                             2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln, (double)esz * c_span,
+                             kind == Kind::D64, 0.0, 2);
+    ka.ksplit = kspl && ka.groups > 1;
     ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
     ka.tmp = nullptr;
     ka.tmp_elems = 0;

@@ -64,11 +64,11 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
 void bind_workspace(Prepared* P, void* ws);
 // Launches a prepared, bound call on its stream.
 int launch_prepared(const Prepared* P);
-// Product groups per C tile (1, 2, 4, 8) for a grid of `tiles` workgroups:
-// `requested` when > 0, else chosen by a time model from the leaf flops,
-// the bytes of C, the leaf's MFMA peak (0: FP64 or FP32 by `fp64`) and the
-// kernel's resident workgroups per CU.
-int split_groups(int requested, int R, long tiles, double leaf_flops, double c_bytes, bool fp64, double peak,
+// Groups per C tile (1, 2, 4, 8; at most max_groups) for a grid of `tiles`
+// workgroups: `requested` when > 0, else chosen by a time model from the
+// leaf flops, the bytes of C, the leaf's MFMA peak (0: FP64 or FP32 by
+// `fp64`) and the kernel's resident workgroups per CU.
+int split_groups(int requested, int max_groups, long tiles, double leaf_flops, double c_bytes, bool fp64, double peak,
                  int wg_per_cu);
 // The shared classical (1x1x1, R = 1) algorithm.
 fmmhip_alg_t classical_alg();

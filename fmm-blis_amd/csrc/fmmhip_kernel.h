@@ -97,7 +97,8 @@ struct KArgs {
     // taking products [g*R/groups, (g+1)*R/groups) and scattering into its
     // own temporary tmp + g*tmp_elems (C's strides; zeroed beforehand, reduced
     // into C afterwards), first writers from bit gbit of the entries' flags.
-    int groups, gbit;
+    // ksplit: the groups split the k-steps instead (R == 1, classical)
+    int groups, gbit, ksplit;
     void* tmp;
     long tmp_elems;
     const int* ptr;              // u, v, w CSR pointers, 3*(R+1)
@@ -320,7 +321,10 @@ __device__ __forceinline__ void epi_block(TO* __restrict__ cb, int rs32, int ero
 // One BM x BN tile position (tm, tn) of the leaf, all R products: the
 // body of both the one-problem kernel and the batched persistent kernel.
 // sA/sB are the workgroup's double-buffered LDS tiles.
-template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
+// KSPL: the groups split the k-steps (a.ksplit; instantiated apart, so that
+// the other kernels keep their registers).
+template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL,
+          bool KSPL = false>
 __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int grp, TL (*sA)[lds_tile<TL>()],
                                           TL (*sB)[lds_tile<TL>()])
 {
@@ -341,12 +345,16 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
     const int R = a.R;
     const bool split = a.groups > 1;
     TO* __restrict__ Cb = split ? (TO*)a.tmp + (long)grp * a.tmp_elems : (TO*)a.C;
-    const int r_begin = (int)((long)grp * R / a.groups), r_end = (int)((long)(grp + 1) * R / a.groups);
+    constexpr bool ks = KSPL;
+    const int r_begin = ks ? 0 : (int)((long)grp * R / a.groups);
+    const int r_end = ks ? R : (int)((long)(grp + 1) * R / a.groups);
     const int* __restrict__ uptr = a.ptr;
     const int* __restrict__ vptr = a.ptr + (R + 1);
     const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
     const fmm_entry_t* __restrict__ ent = a.ent;
     const int nk = (a.lk + BK - 1) / BK;
+    const int kt_begin = ks ? (int)((long)grp * nk / a.groups) : 0;
+    const int kt_end = ks ? (int)((long)(grp + 1) * nk / a.groups) : nk;
     const long lda = a.lda, ldb = a.ldb;
 
     // tiles inside every part need no bounds checks
@@ -373,24 +381,26 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
 #pragma unroll
             for (int j = 0; j < FJ; j++) acc[i][j] = (acc_t){0, 0, 0, 0};
 
-        // prologue: stage k-step 0 (all groups, one after another)
+        // prologue: stage the first k-step (all groups, one after another)
         {
-            const bool ia = a_rows_in && BK <= a.min_k, ib = b_cols_in && BK <= a.min_k;
+            const int kp = kt_begin * BK;
+            const bool kpin = kp + BK <= a.min_k;
+            const bool ia = a_rows_in && kpin, ib = b_cols_in && kpin;
             if (ia && ib) {
-                stA.template load_first<VEC>(Ab, ea0, lda, m0, 0, true);
-                stB.template load_first<VEC>(Bb, eb0, ldb, n0, 0, true);
+                stA.template load_first<VEC>(Ab, ea0, lda, m0, kp, true);
+                stB.template load_first<VEC>(Bb, eb0, ldb, n0, kp, true);
             } else {
-                stA.template load_first<VEC>(Ab, ea0, lda, m0, 0, ia);
-                stB.template load_first<VEC>(Bb, eb0, ldb, n0, 0, ib);
+                stA.template load_first<VEC>(Ab, ea0, lda, m0, kp, ia);
+                stB.template load_first<VEC>(Bb, eb0, ldb, n0, kp, ib);
             }
             stA.scale_first(ea0.coef);
             stB.scale_first(eb0.coef);
             for (int g = 1; g < nu; g++) {
-                stA.template load_tmp<VEC>(Ab, ent[u0 + g], lda, m0, 0, ia);
+                stA.template load_tmp<VEC>(Ab, ent[u0 + g], lda, m0, kp, ia);
                 stA.add_tmp(ent[u0 + g].coef);
             }
             for (int g = 1; g < nv; g++) {
-                stB.template load_tmp<VEC>(Bb, ent[v0 + g], ldb, n0, 0, ib);
+                stB.template load_tmp<VEC>(Bb, ent[v0 + g], ldb, n0, kp, ib);
                 stB.add_tmp(ent[v0 + g].coef);
             }
             stA.template store<TL>(sA[0]);
@@ -404,9 +414,9 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
         // arrived is folded into the sum and group g+2 is issued, so each
         // part's load is covered by a quarter of the MFMA block.
         constexpr int KS = BK / 4;
-        for (int kt = 0; kt < nk; kt++) {
-            const int cur = kt & 1;
-            const bool more = kt + 1 < nk;
+        for (int kt = kt_begin; kt < kt_end; kt++) {
+            const int cur = (kt - kt_begin) & 1;
+            const bool more = kt + 1 < kt_end;
             const int k0 = (kt + 1) * BK;
             const bool kin = k0 + BK <= a.min_k;
             const bool ia = a_rows_in && kin, ib = b_cols_in && kin;
@@ -516,7 +526,8 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
     }
 }
 
-template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL>
+template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL,
+          bool KSPL = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>::NT),
                           amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
 void fmm_gemm_kernel(KArgs a)
@@ -528,7 +539,7 @@ void fmm_gemm_kernel(KArgs a)
     const int grp = (int)blockIdx.x / tiles;
     int tm, tn;
     tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
-    gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL>(a, tm, tn, grp, sA, sB);
+    gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL, KSPL>(a, tm, tn, grp, sA, sB);
 }
 
 // Batched persistent kernel: a fixed grid of workgroups takes work items
@@ -576,6 +587,13 @@ int launch_layouts(const KArgs& ka, bool a_mn, bool b_mn, bool c_col, int nwg, h
     hipLaunchKernelGGL((fmm_gemm_kernel<TI, TL, TO, SHAPE, VEC, AM, BMN, CC>), dim3(nwg),                \
                        dim3(Cfg<SHAPE>::NT), 0, st, ka)
     const int sel = (a_mn ? 4 : 0) | (b_mn ? 2 : 0) | (c_col ? 1 : 0);
+    if (ka.ksplit) {
+        // the k-split is prepared for column-major A, B and C only
+        if (sel != 5) return FMMHIP_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((fmm_gemm_kernel<TI, TL, TO, SHAPE, VEC, true, false, true, true>), dim3(nwg),
+                           dim3(Cfg<SHAPE>::NT), 0, st, ka);
+        return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+    }
     switch (sel) {
     case 0: FMMK_GO(false, false, false); break;
     case 1: FMMK_GO(false, false, true); break;

@@ -299,7 +299,7 @@ extern "C" int fmmhip_server_create(fmmhip_server_t* out, size_t hbm_budget, int
     if (rc) { fmmhip_server_destroy(s); return rc; }
     // no product split: concurrent requests fill the device, and a split
     // would add up to 8 C-sized temporaries to every request's arena share
-    for (int c = 1; c < kCands; c++) s->algs[c]->split = 1;
+    for (int c = 0; c < kCands; c++) s->algs[c]->split = 1;
     for (int c = 0; c < kCands; c++) s->list_cap = std::max(s->list_cap, (desc_bound(s->algs[c]) + 255) & ~(size_t)255);
     if (hipMalloc(&s->dev, hbm_budget) != hipSuccess) {
         (void)hipGetLastError();

@@ -226,12 +226,13 @@ int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
  * for multi-part algorithms when the workspace fits in half of the free
  * device memory).  The W scatter is fused into the kernel in both modes. */
 int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
-/* Product split: 1 = each 128 x 128 tile of C is one workgroup looping over
- * all R products; 2, 4, 8 = that many workgroups per tile, each taking a
- * contiguous range of products and scattering into its own zeroed copy of
- * C, summed into C in group order by a second kernel (deterministic, but
- * rounded differently from 1); -1 = chosen per call (default) from the
- * tile count so that the grid fills whole waves of the device.  No
+/* Split: 1 = each 128 x 128 tile of C is one workgroup looping over all R
+ * products; 2, 4, 8 = that many workgroups per tile, each taking a
+ * contiguous range of products (the classical algorithm, R = 1: a
+ * contiguous range of at least 8 k-steps of 16) and scattering into its own
+ * zeroed copy of C, summed into C in group order by a second kernel
+ * (deterministic, but rounded differently from 1); -1 = chosen per call
+ * (default) by a time model of the wave fill and the copies' traffic.  No
  * counterpart in the reference: its CPU loop over products is serial per
  * thread (frame/base/bli_strassen_ab.c:248); this is the GPU's occupancy knob. */
 int  fmmhip_alg_set_split(fmmhip_alg_t alg, int groups);

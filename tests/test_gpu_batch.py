@@ -86,10 +86,9 @@ def expected(p):
 def alg_for(cand):
     """The single-call algorithm the batch runs for `cand` (pre-summed, no
     product split)."""
-    if cand == 0:
-        return F.Alg(classical=True)
-    a = F.Alg([F.strassen()] * cand)
-    a.set_pack(1)
+    a = F.Alg(classical=True) if cand == 0 else F.Alg([F.strassen()] * cand)
+    if cand > 0:
+        a.set_pack(1)
     a.set_split(1)
     return a
 

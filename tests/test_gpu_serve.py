@@ -58,7 +58,7 @@ def alone(cand, dt, m, n, k, alpha, dA, dB, beta, dC):
     alg = F.Alg(classical=True) if cand == 0 else F.Alg([F.strassen()] * cand)
     if cand > 0:
         alg.set_pack(1)
-        alg.set_split(1)
+    alg.set_split(1)
     (F.dgemm_fmm if dt == "f8" else F.sgemm_fmm)(alg, m, n, k, alpha, dA, dB, beta, dC)
     F.sync()
     return dC.to_numpy()

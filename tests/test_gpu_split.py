@@ -1,6 +1,7 @@
 """GPU parity of the product split (fmmhip_alg_set_split): G workgroups per
-C tile, each taking a contiguous range of the R products and scattering
-into its own zeroed temporary of C, then C := beta*C + T_0 + ... + T_{G-1}.
+C tile, each taking a contiguous range of the R products (of the k-steps
+for the classical algorithm) and scattering into its own zeroed temporary
+of C, then C := beta*C + T_0 + ... + T_{G-1}.
 
 Bars (stated per test):
   * integer-valued inputs: every split (1, 2, 4, 8), pack mode, level count
@@ -138,3 +139,60 @@ def test_split_fp32():
         err = O.normalized_error(dC.to_numpy().astype(np.float64), ref, A.astype(np.float64), B.astype(np.float64))
         say(f"fp32 split {g} err {err:.3e}")
         assert err < 1e-7
+
+
+def test_classical_k_split_exact_and_repeatable():
+    """The classical algorithm (R = 1) splits the k-steps instead: G groups
+    of at least 8 k-steps each, zeroed temporaries and the ordered reduction.
+    Integer inputs are exact for every G; uniform inputs stay within 1e-15
+    and a repeated call is bit-identical; beta = 0 does not read C."""
+    m, n, k = 300, 260, 1100                   # 69 k-steps: G up to 8
+    A, B, C = O.integers(m, k, 31, 0, 4), O.integers(k, n, 31, 1, 4), O.integers(m, n, 31, 2, 4)
+    for alpha, beta in [(1.0, 1.0), (2.0, -3.0), (0.5, 0.0)]:
+        want = O.dgemm(alpha, A, B, beta, C.copy(order="F"))
+        Cn = C.copy(order="F")
+        if beta == 0.0:
+            Cn[:] = np.nan
+        for g in (1, 2, 4, 8):
+            a = F.Alg(classical=True)
+            a.set_split(g)
+            dA, dB, dC = dev(A), dev(B), dev(Cn)
+            F.dgemm_fmm(a, m, n, k, alpha, dA, dB, beta, dC)
+            F.sync()
+            say("classical k-split", g, alpha, beta)
+            assert np.array_equal(dC.to_numpy(), want), (g, alpha, beta)
+    U, V, W = O.uniform(m, k, 32, 0), O.uniform(k, n, 32, 1), O.uniform(m, n, 32, 2)
+    ref = O.dgemm(1.0, U, V, 1.0, W.copy(order="F"))
+    a = F.Alg(classical=True)
+    a.set_split(4)
+    outs = []
+    for _ in range(2):
+        dU, dV, dW = dev(U), dev(V), dev(W)
+        F.dgemm_fmm(a, m, n, k, 1.0, dU, dV, 1.0, dW)
+        F.sync()
+        outs.append(dW.to_numpy())
+    assert O.normalized_error(outs[0], ref, U, V) < 1e-15
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_classical_k_split_fp32_and_other_layouts():
+    """FP32 operands take the k-split too; a row-stored C does not (the
+    split is prepared for column-major operands only) and is still exact."""
+    m, n, k = 200, 180, 900
+    A, B, C = (np.asfortranarray(X.astype(np.float32)) for X in
+               (O.integers(m, k, 33, 0, 2), O.integers(k, n, 33, 1, 2), O.integers(m, n, 33, 2, 2)))
+    want = (C.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64)).astype(np.float32)
+    a = F.Alg(classical=True)
+    a.set_split(8)
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    F.sgemm_fmm(a, m, n, k, 1.0, dA, dB, 1.0, dC)
+    F.sync()
+    assert np.array_equal(dC.to_numpy(), want)
+    A64, B64, C64 = (X.astype(np.float64) for X in (A, B, C))
+    want64 = O.dgemm(1.0, A64, B64, 1.0, C64.copy(order="F"))
+    a.set_split(4)
+    dA, dB, dC = dev(np.asfortranarray(A64)), dev(np.asfortranarray(B64)), dev(np.asfortranarray(C64.T))
+    F.check(F.lib().fmmhip_dgemm_fmm(a.h, 0, m, n, k, 1.0, dA.ptr, 1, m, dB.ptr, 1, k, 1.0, dC.ptr, n, 1, None),
+            "dgemm_fmm row-major C")
+    F.sync()
+    assert np.array_equal(dC.to_numpy().T, want64)
