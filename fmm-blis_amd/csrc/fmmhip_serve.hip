@@ -298,8 +298,11 @@ extern "C" int fmmhip_server_create(fmmhip_server_t* out, size_t hbm_budget, int
     if (!rc) rc = fmmhip_alg_create(&s->algs[2], two, 2);
     if (rc) { fmmhip_server_destroy(s); return rc; }
     // no product split: concurrent requests fill the device, and a split
-    // would add up to 8 C-sized temporaries to every request's arena share
-    for (int c = 0; c < kCands; c++) s->algs[c]->split = 1;
+    // would add up to 8 C-sized temporaries to every request's arena share;
+    // the classical candidate keeps its k-split, taken only by requests too
+    // small to occupy half the CUs (small temporaries; the choice depends
+    // on the request alone, so a result is the same alone and in the mix)
+    for (int c = 1; c < kCands; c++) s->algs[c]->split = 1;
     for (int c = 0; c < kCands; c++) s->list_cap = std::max(s->list_cap, (desc_bound(s->algs[c]) + 255) & ~(size_t)255);
     if (hipMalloc(&s->dev, hbm_budget) != hipSuccess) {
         (void)hipGetLastError();

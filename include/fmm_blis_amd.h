@@ -310,8 +310,9 @@ int fmmhip_tuner_load(const char* path);
  * on the device.  C == NULL: the server allocates C (ldc = m) in its arena
  * (beta must be 0); keep = 1 keeps it until fmmhip_server_release, keep = 0
  * drops it at completion (latency/throughput only).  cand: -1 tuner, 0
- * classical, 1/2 levels of Strassen (pre-summed, no product split, so a
- * request's result does not depend on what else is in flight).  arrival_s:
+ * classical, 1/2 levels of Strassen (pre-summed, no product split; the
+ * classical k-split as for a single call: a request's result does not
+ * depend on what else is in flight).  arrival_s:
  * the time, on the server's
  * clock (fmmhip_server_now), at which the request arrives; earlier means now. */
 typedef struct {

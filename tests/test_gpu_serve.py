@@ -53,12 +53,13 @@ def fetch(ptr, m, n, ld, dtype):
 
 
 def alone(cand, dt, m, n, k, alpha, dA, dB, beta, dC):
-    """The single-call equivalent of a served request: the server's
-    candidates run pre-summed and without product split."""
+    """The single-call equivalent of a served request: the server's FMM
+    candidates run pre-summed and without product split; the classical one
+    keeps its per-call k-split choice."""
     alg = F.Alg(classical=True) if cand == 0 else F.Alg([F.strassen()] * cand)
     if cand > 0:
         alg.set_pack(1)
-    alg.set_split(1)
+        alg.set_split(1)
     (F.dgemm_fmm if dt == "f8" else F.sgemm_fmm)(alg, m, n, k, alpha, dA, dB, beta, dC)
     F.sync()
     return dC.to_numpy()
