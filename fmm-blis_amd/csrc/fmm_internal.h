@@ -100,6 +100,27 @@ typedef struct fmmhip_presum_seg_s {
 int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff, int nblocks,
                         int parts, void* stream);
 
+/* W stage of the products-materialised pack mode (fmmhip_wsum.hip): for
+ * every C part c, C_c := beta C_c + sum_r (alpha W[c][r]) M_r over the
+ * part's extent, r ascending, from the slots M_r the MFMA kernel stored. */
+typedef struct fmmhip_wsum_s {
+    const void* M;              /* R slots, slot r at M + off of its term */
+    long ldm;                   /* slot leading dimension (along s) */
+    int swap;                   /* 0: c runs along C's rows (column-major C), 1: along its columns */
+    int Lc, Ls;                 /* leaf extents along c and s */
+    void* C;
+    long rsC, csC;
+    double alpha, beta;
+    int np;                     /* C parts */
+    /* device part table: off, rows, cols of C part p; `first` = index of its
+     * first term, `part` = number of its terms */
+    const fmm_entry_t* parts;
+    /* device terms, per part in ascending product order: off = element
+     * offset of the product's slot, coef = W[c][r] */
+    const fmm_entry_t* terms;
+} fmmhip_wsum_t;
+int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
+
 /* Summation stage of the pre-summed pack mode (fmmhip_presum.hip). */
 int fmmhip_presum_d(const double* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,
                     int Lc, int Ls, int ldo, double* S, void* stream);

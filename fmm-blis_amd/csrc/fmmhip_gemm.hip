@@ -36,6 +36,8 @@ struct DevDesc {
     DevList fused;               // every nonzero part of every product
     DevList presum;              // one part per product (the workspace leaf), W as in fused
     bool has_presum = false;
+    DevList mat;                 // as presum, W = one full-leaf slot per product, then the W-stage tables
+    bool has_mat = false;
     unsigned long last_use = 0;
 };
 
@@ -65,6 +67,7 @@ void free_desc(DevDesc* d)
 {
     free_list(d->fused);
     free_list(d->presum);
+    free_list(d->mat);
     delete d;
 }
 
@@ -148,10 +151,92 @@ int build_presum_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const
     return FMMHIP_SUCCESS;
 }
 
+// Slot layout of the products-materialised mode: R leaves of lm x ln,
+// contiguous along C's contiguous dimension (rows unless C is row-major).
+struct SlotLayout {
+    int swap;                    // 1: slots row-major (C row-major)
+    int Lc, Ls;
+    long ldm, slot;
+};
+
+SlotLayout slot_layout(const fmm_geom_t& g)
+{
+    SlotLayout sl;
+    sl.swap = (g.csC == 1 && g.rsC != 1) ? 1 : 0;
+    sl.Lc = (int)(sl.swap ? g.ln : g.lm);
+    sl.Ls = (int)(sl.swap ? g.lm : g.ln);
+    sl.ldm = sl.Lc;
+    sl.slot = sl.ldm * sl.Ls;
+    return sl;
+}
+
+// The products-materialised list: the pre-summed A and B leaves, one W
+// entry per product (its slot, coefficient 1, first writer), then the W
+// stage's tables: the mt*nt C parts (off, rows, cols; `first` = index of
+// the part's first term, `part` = its term count) and the terms (slot
+// offset, W coefficient), per part in ascending product order -- the order
+// in which the fused epilogue updates that part.
+int build_mat_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const DevList& fused, const DevList& presum,
+                   DevList& out)
+{
+    const int R = alg->R;
+    const int npc = alg->mt * alg->nt;
+    const SlotLayout sl = slot_layout(g);
+    out.nent = 3L * R + npc + alg->nnz_w;
+    out.ptr_h = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
+    out.ent_h = (fmm_entry_t*)calloc((size_t)out.nent + 1, sizeof(fmm_entry_t));
+    if (!out.ptr_h || !out.ent_h) return FMMHIP_ERR_NOMEM;
+    for (int r = 0; r <= R; r++) {
+        out.ptr_h[r] = r;
+        out.ptr_h[(R + 1) + r] = R + r;
+        out.ptr_h[2 * (R + 1) + r] = 2 * R + r;
+    }
+    memcpy(out.ent_h, presum.ent_h, sizeof(fmm_entry_t) * 2 * (size_t)R);
+    for (int r = 0; r < R; r++) {
+        fmm_entry_t& ew = out.ent_h[2 * R + r];
+        ew.off = (int64_t)r * sl.slot;
+        ew.rows = (int32_t)g.lm;
+        ew.cols = (int32_t)g.ln;
+        ew.coef = 1.0;
+        ew.first = -1;           // first writer in every group: a plain store
+        ew.part = r;
+    }
+    fmm_entry_t* parts = out.ent_h + 3 * R;
+    fmm_entry_t* terms = parts + npc;
+    const fmm_entry_t* wf = fused.ent_h + alg->nnz_u + alg->nnz_v;
+    const int32_t* wptr = fused.ptr_h + 2 * (R + 1);
+    const long w0 = alg->nnz_u + alg->nnz_v;
+    std::vector<int> count(npc, 0);
+    for (long e = 0; e < alg->nnz_w; e++) {
+        const int c = wf[e].part;
+        if (c < 0 || c >= npc) return FMMHIP_ERR_INVALID;
+        count[c]++;
+        parts[c].off = wf[e].off;
+        parts[c].rows = wf[e].rows;
+        parts[c].cols = wf[e].cols;
+    }
+    int at = 0;
+    for (int c = 0; c < npc; c++) {
+        if (count[c] == 0) return FMMHIP_ERR_INVALID;   // a C part no product reaches
+        parts[c].first = at;
+        parts[c].part = 0;
+        at += count[c];
+    }
+    for (int r = 0; r < R; r++)
+        for (long e = wptr[r] - w0; e < wptr[r + 1] - w0; e++) {
+            fmm_entry_t& pc = parts[wf[e].part];
+            fmm_entry_t& t = terms[pc.first + pc.part++];
+            t.off = (int64_t)r * sl.slot;
+            t.coef = wf[e].coef;
+            t.part = r;
+        }
+    return FMMHIP_SUCCESS;
+}
+
 // Returns the cached descriptor for geometry g (building and uploading the
 // fused list, and the pre-summed list when asked, stream-ordered).
-int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool a_mn, bool b_mn, hipStream_t stream,
-             DevDesc** out)
+int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool want_mat, bool a_mn, bool b_mn,
+             hipStream_t stream, DevDesc** out)
 {
     if (!alg->dev_cache) {
         static std::mutex create_mu;
@@ -186,6 +271,12 @@ int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool a_mn,
         if (!rc) rc = upload_list(d->presum, alg->R, stream);
         if (rc) { free_list(d->presum); return rc; }
         d->has_presum = true;
+    }
+    if (want_mat && !d->has_mat) {
+        int rc = build_mat_list(alg, g, d->fused, d->presum, d->mat);
+        if (!rc) rc = upload_list(d->mat, alg->R, stream);
+        if (rc) { free_list(d->mat); return rc; }
+        d->has_mat = true;
     }
     d->last_use = cache->clock;
     *out = d;
@@ -304,6 +395,76 @@ int split_groups(int requested, int max_groups, long tiles, double leaf_flops, d
     return best;
 }
 
+// Groups per tile of the products-materialised mode (no copies of C to
+// pay for): the G <= min(R, 64) that minimises the waves of workgroups
+// times the products of the longest group, ceil(tiles G / slots)
+// ceil(R / G); ties go to fewer groups.
+int mat_groups(int R, long tiles, int slots)
+{
+    int best = 1;
+    double best_t = 0.0;
+    for (int g = 1; g <= std::min(R, 64); g++) {
+        const double t = std::ceil((double)tiles * g / slots) * std::ceil((double)R / g);
+        if (g == 1 || t < best_t) { best = g; best_t = t; }
+    }
+    return best;
+}
+
+// Whether the pre-summed mode is expected to beat the products-materialised
+// one by 3% (the automatic choice otherwise materialises: measured on
+// 222, 333, 444 at 1 and 2 levels from 4096^3 to 16384^3 it is never more
+// than 2% slower and up to 29% faster, its grid filling the device with
+// groups that need no copies of C).  Time model (both share the summation stage): the leaf
+// flops at a sustained MFMA rate divided by the wave fill of each mode's
+// grid (pre-summed: the split its own model picks, copies of C charged;
+// materialised: mat_groups, no copies), plus, pre-summed, the fused W
+// epilogue's read-modify-write of one C leaf per W entry at its measured
+// effective rate (6.8 TB/s, partly hidden behind the MFMAs), or,
+// materialised, the slot writes and the W stage (one slot read per W
+// entry, C read and written once) at 4.9 TB/s.
+bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, size_t esz, double c_span)
+{
+    int cus = 256;
+    fmmhip_device_cu_count(&cus);
+    const int slots = 2 * cus;
+    const long tiles = ((g.lm + BM - 1) / BM) * ((g.ln + BN - 1) / BN);
+    const int R = alg->R;
+    const double leaf_flops = 2.0 * R * (double)g.lm * (double)g.lk * (double)g.ln;
+    const double rate = (kind == Kind::D64 ? 78.6e12 : 157.3e12) * 0.87;
+    const double leaf_c = (double)esz * (double)g.lm * (double)g.ln;
+    const int g1 = split_groups(split, R, tiles, leaf_flops, (double)esz * c_span, kind == Kind::D64, 0.0, 2);
+    const double items1 = (double)tiles * g1;
+    const double fill1 = items1 / (std::ceil(items1 / slots) * slots);
+    double t1 = leaf_flops / (rate * fill1) + (double)alg->nnz_w * leaf_c * 2.0 / 6.8e12;
+    if (g1 > 1) t1 += (double)esz * c_span * (2.0 * g1 + 2.0) / 4.0e12;
+    const int g2 = split > 0 ? std::min(split, R) : mat_groups(R, tiles, slots);
+    const double fill2 = ((double)tiles * R / slots) /
+                         (std::ceil((double)tiles * g2 / slots) * std::ceil((double)R / g2));
+    const double t2 = leaf_flops / (rate * std::min(fill2, 1.0)) +
+                      ((double)R * leaf_c + (double)alg->nnz_w * leaf_c + 2.0 * (double)esz * c_span) / 4.9e12;
+    return t1 < 0.97 * t2;
+}
+
+// The pack mode of one call: 0 fused, 1 pre-summed, 2 pre-summed with the
+// products materialised.  Asked for (pack >= 0; materialising needs
+// several products and the one-problem path, whose lists it builds), or
+// automatic: pre-summed for multi-part algorithms when its workspace fits
+// in half of the free device memory, materialised as well when the slots
+// fit too and the pre-summed mode is not expected to win (presum_wins).
+int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int split, size_t esz, size_t ws_bytes,
+                 double c_span, bool one_problem)
+{
+    const bool can_mat = alg->R > 1 && one_problem;
+    if (pack >= 0) return pack == 2 ? (can_mat ? 2 : 1) : pack;
+    if (alg->max_nnz_u <= 1 && alg->max_nnz_v <= 1) return 0;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess || !(ws_bytes < kWsFractionOfFree * (double)fr)) return 0;
+    if (!can_mat) return 1;
+    const size_t slots = esz * (size_t)g.lm * (size_t)g.ln * (size_t)alg->R;
+    if (!(ws_bytes + slots < kWsFractionOfFree * (double)fr)) return 1;
+    return presum_wins(alg, kind, g, split, esz, c_span) ? 1 : 2;
+}
+
 int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
                  int split, hipStream_t st, Prepared* P, DescSource* src)
@@ -348,11 +509,10 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     const size_t bytesA = esz * wa.elems(alg->R);
     const size_t offB = (bytesA + 255) & ~(size_t)255;
     const size_t ws_bytes = offB + esz * wb.elems(alg->R) + 256;
-    bool presum = pack == 1;
-    if (pack < 0 && multi) {
-        size_t fr = 0, tot = 0;
-        presum = hipMemGetInfo(&fr, &tot) == hipSuccess && ws_bytes < kWsFractionOfFree * (double)fr;
-    }
+    const int mode = resolve_pack(alg, kind, g, pack, split, esz, ws_bytes,
+                                  (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0, !src);
+    const bool wmat = mode == 2;
+    const bool presum = mode >= 1;
 
     DescView dv;
     memset(&dv, 0, sizeof(dv));
@@ -361,7 +521,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         rc = src->get(alg, g, presum, a_mn, b_mn, st, &dv);
     } else {
         DevDesc* d = nullptr;
-        rc = get_desc(alg, g, presum, a_mn, b_mn, st, &d);
+        rc = get_desc(alg, g, presum, wmat, a_mn, b_mn, st, &d);
         if (!rc) {
             dv.fused_ptr = d->fused.dptr();
             dv.fused_ent = d->fused.dent();
@@ -369,6 +529,10 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
             if (presum) {
                 dv.presum_ptr = d->presum.dptr();
                 dv.presum_ent = d->presum.dent();
+            }
+            if (wmat) {
+                dv.mat_ptr = d->mat.dptr();
+                dv.mat_ent = d->mat.dent();
             }
         }
     }
@@ -439,6 +603,44 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     fmmhip_device_cu_count(&cus);
     const bool kspl = alg->R == 1 && a_mn && !b_mn && rsC == 1 && 2 * tiles <= cus;
     const long ksteps = (g.lk + fmmk::BK - 1) / fmmk::BK;
+    if (wmat) {
+        // every product writes its own slot: the groups need no copies of C
+        // and are chosen for the wave fill alone
+        const SlotLayout sl = slot_layout(g);
+        P->wmat = true;
+        P->m_off = (P->ws_bytes + 255) & ~(size_t)255;
+        P->ws_bytes = P->m_off + esz * (size_t)sl.slot * (size_t)alg->R;
+        ka.C = (void*)(uintptr_t)P->m_off;
+        ka.rsC = sl.swap ? sl.ldm : 1;
+        ka.csC = sl.swap ? 1 : sl.ldm;
+        ka.min_cm = (int)g.lm; ka.min_cn = (int)g.ln;
+        ka.alpha = 1.0;
+        ka.beta = 0.0;
+        ka.ptr = dv.mat_ptr;
+        ka.ent = dv.mat_ent;
+        P->c_col = !sl.swap;
+        ka.groups = split > 0 ? std::min(split, alg->R) : mat_groups(alg->R, tiles, 2 * cus);
+        ka.ksplit = false;
+        ka.gbit = 0;
+        ka.tmp = ka.C;           // group g's Cb: the slots (no per-group copy)
+        ka.tmp_elems = 0;
+        fmmhip_wsum_t& w = P->wsum;
+        w.M = ka.C;
+        w.ldm = sl.ldm;
+        w.swap = sl.swap;
+        w.Lc = sl.Lc; w.Ls = sl.Ls;
+        w.C = C;
+        w.rsC = rsC; w.csC = csC;
+        w.alpha = alpha;
+        w.beta = beta;
+        w.np = alg->mt * alg->nt;
+        w.parts = dv.mat_ent + 3 * alg->R;
+        w.terms = w.parts + w.np;
+        P->nwg = tiles * ka.groups;
+        if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+        if (tiles <= 0) P->nothing = true;
+        return FMMHIP_SUCCESS;
+    }
     ka.groups = split_groups(split, alg->R == 1 ? (kspl ? (int)std::min(ksteps / 8, 8L) : 1) : alg->R, tiles,
                              2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln, (double)esz * c_span,
                              kind == Kind::D64, 0.0, 2);
@@ -462,7 +664,13 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
 void bind_workspace(Prepared* P, void* ws)
 {
     char* base = (char*)ws;
-    if (P->ka.groups > 1) P->ka.tmp = base + P->tmp_off;
+    if (P->wmat) {
+        P->ka.C = base + P->m_off;
+        P->ka.tmp = P->ka.C;
+        P->wsum.M = P->ka.C;
+    } else if (P->ka.groups > 1) {
+        P->ka.tmp = base + P->tmp_off;
+    }
     if (!P->presum) return;
     P->segA.S = base + (uintptr_t)P->segA.S;
     P->segB.S = base + (uintptr_t)P->segB.S;
@@ -489,7 +697,8 @@ int launch_prepared(const Prepared* P)
     if (timed) tm_record(&tc.e0, st);
     int rc = FMMHIP_SUCCESS;
     const bool dbl = P->kind != Kind::S32;
-    if (P->ka.groups > 1) {
+    const bool split = P->ka.groups > 1 && !P->wmat;
+    if (split) {
         rc = fmmhip_split_zero(dbl, P->ka.tmp, P->ka.tmp_elems, P->ka.groups, st);
         if (rc) return rc;
     }
@@ -521,11 +730,12 @@ int launch_prepared(const Prepared* P)
     case Kind::S32: rc = fmmk_launch_s32(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
     }
     if (timed) tm_record(&tc.e2, st);
-    if (!rc && P->ka.groups > 1)
+    if (!rc && split)
         rc = fmmhip_split_reduce(dbl, P->C, P->m, P->n, P->rsC, P->csC, P->beta, P->ka.tmp, P->ka.tmp_elems,
                                  P->ka.groups, st);
+    if (!rc && P->wmat) rc = fmmhip_wsum(dbl, &P->wsum, st);
     if (timed) {
-        if (P->ka.groups > 1) tm_record(&tc.e3, st);
+        if (split || P->wmat) tm_record(&tc.e3, st);
         std::lock_guard<std::mutex> lk(g_tm_mu);
         g_tm.push_back(tc);
     }
@@ -551,7 +761,7 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
     if (P.ws_bytes) {
         void* ws = workspace_get(st, P.ws_bytes);
         if (!ws) {
-            if (pack == 1 || alg->split > 1) return FMMHIP_ERR_NOMEM;
+            if (pack >= 1 || alg->split > 1) return FMMHIP_ERR_NOMEM;
             // automatic choices fall back to the fused mode without split
             rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, 0, 1, st, &P);
             if (rc) return rc;
@@ -565,7 +775,7 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
 int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int** dptr, const fmm_entry_t** dent)
 {
     DevDesc* d = nullptr;
-    int rc = get_desc(alg, g, false, true, false, st, &d);
+    int rc = get_desc(alg, g, false, false, true, false, st, &d);
     if (rc) return rc;
     *dptr = d->fused.dptr();
     *dent = d->fused.dent();
@@ -672,8 +882,29 @@ extern "C" int fmmhip_alg_set_split(fmmhip_alg_t alg, int groups)
 
 extern "C" int fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack)
 {
-    if (!alg || pack < -1 || pack > 1) return FMMHIP_ERR_INVALID;
+    if (!alg || pack < -1 || pack > 2) return FMMHIP_ERR_INVALID;
     alg->pack = pack;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, long n, long k, int* pack)
+{
+    if (!alg || !pack || m < 0 || n < 0 || k < 0) return FMMHIP_ERR_INVALID;
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6) { *pack = 1; return FMMHIP_SUCCESS; }
+    const Kind kind = leaf == FMMHIP_LEAF_F32 ? Kind::D_F32LEAF : Kind::D64;
+    fmm_geom_t g;
+    memset(&g, 0, sizeof(g));
+    g.m = m; g.n = n; g.k = k;
+    g.lm = (m + alg->mt - 1) / alg->mt;
+    g.lk = (k + alg->kt - 1) / alg->kt;
+    g.ln = (n + alg->nt - 1) / alg->nt;
+    g.rsA = 1; g.csA = m; g.rsB = 1; g.csB = k; g.rsC = 1; g.csC = m;
+    const WsLayout wa = ws_layout_a(g, true), wb = ws_layout_b(g, false);
+    const size_t offB = (8 * wa.elems(alg->R) + 255) & ~(size_t)255;
+    const size_t ws_bytes = offB + 8 * wb.elems(alg->R) + 256;
+    *pack = resolve_pack(alg, kind, g, alg->pack, alg->split, 8, ws_bytes, (double)m * (double)n, true);
     return FMMHIP_SUCCESS;
 }
 

@@ -27,6 +27,9 @@ struct Prepared {
     size_t ws_bytes;             // workspace: summation leaves, then the split temporaries
     fmmhip_presum_seg_t segA, segB;
     size_t tmp_off;              // split temporaries: workspace offset (ka.groups > 1)
+    bool wmat;                   // products materialised: kernel into slots, then the W stage
+    size_t m_off;                // slots: workspace offset
+    fmmhip_wsum_t wsum;          // the W stage's arguments (slots bound with the workspace)
 };
 
 // Device coefficient lists of one call: the fused list (every nonzero part
@@ -38,6 +41,8 @@ struct DescView {
     const fmm_entry_t* fused_ent_h;
     const int* presum_ptr;
     const fmm_entry_t* presum_ent;
+    const int* mat_ptr;          // products-materialised list (one-problem path only)
+    const fmm_entry_t* mat_ent;
 };
 
 // Where prepare_call takes the lists from; without one, from the
@@ -55,8 +60,9 @@ size_t desc_bound(fmmhip_alg_t alg);
 int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, void* host, const void* dev,
               size_t cap, size_t* used, DescView* v);
 
-// Resolves one call (pack: 0 fused, 1 pre-summed, -1 automatic; split:
-// product groups per tile, 1, 2, 4 or 8, -1 chosen from the tile count).
+// Resolves one call (pack: 0 fused, 1 pre-summed, 2 pre-summed with the
+// products materialised, -1 automatic; split: product groups per tile, 1,
+// 2, 4 or 8, -1 chosen from the tile count; with pack 2 any count up to R).
 int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
                  int split, hipStream_t st, Prepared* P, DescSource* src = nullptr);

@@ -117,6 +117,7 @@ _SIGS = {
     "fmmhip_alg_brent_violations": (_L, [_P]),
     "fmmhip_alg_set_shape": (_I, [_P, _I]),
     "fmmhip_alg_set_pack": (_I, [_P, _I]),
+    "fmmhip_alg_pack_for": (_I, [_P, _I, _L, _L, _L, ctypes.POINTER(_I)]),
     "fmmhip_alg_set_split": (_I, [_P, _I]),
     "fmmhip_workspace_release": (_I, []),
     "fmmhip_timing_enable": (_I, [_I]),
@@ -238,6 +239,13 @@ class Alg:
 
     def set_pack(self, pack):
         check(lib().fmmhip_alg_set_pack(self.h, pack), "alg_set_pack")
+
+    def pack_for(self, m, n, k, leaf=0):
+        """Pack mode (0 fused, 1 pre-summed, 2 products materialised) a
+        column-major FP64 call of m x n x k would use now."""
+        p = ctypes.c_int(-1)
+        check(lib().fmmhip_alg_pack_for(self.h, leaf, m, n, k, ctypes.byref(p)), "alg_pack_for")
+        return p.value
 
     def set_split(self, groups):
         """Product groups per C tile: 1, 2, 4, 8, or -1 (chosen per call)."""

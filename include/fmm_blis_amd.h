@@ -218,14 +218,30 @@ long fmmhip_alg_brent_violations(fmmhip_alg_t alg);
  * product loaded ahead), 1 = 256 threads (two parts ahead), -1 = chosen per
  * call (default).  Used by the autotuner. */
 int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
-/* How the U/V-weighted operand sums are formed: 0 = fused (streamed from the
- * operands into LDS inside the MFMA kernel, no workspace), 1 = pre-summed
- * (an HBM-bandwidth pass writes every S_r and T_r once to a per-stream
- * device workspace of R*(leaf(A) + leaf(B)) elements; the MFMA kernel then
- * reads one part per product), -1 = chosen per call (default: pre-summed
- * for multi-part algorithms when the workspace fits in half of the free
- * device memory).  The W scatter is fused into the kernel in both modes. */
+/* How the U/V-weighted operand sums are formed and the W-weighted products
+ * applied: 0 = fused (operand sums streamed from the operands into LDS
+ * inside the MFMA kernel, W scatter in its epilogue, no workspace); 1 =
+ * pre-summed (an HBM-bandwidth pass writes every S_r and T_r once to a
+ * per-stream device workspace of R*(leaf(A) + leaf(B)) elements; the MFMA
+ * kernel reads one part per product and scatters with W in its epilogue);
+ * 2 = pre-summed with the products materialised (the MFMA kernel stores
+ * every M_r once into R more leaves of C's size, leaf(C) each, and a
+ * bandwidth pass forms each C part from the products it needs: one pass
+ * per part instead of one read-modify-write of C per (product, part), and
+ * any number of workgroups per tile, up to R, without copies of C; the
+ * same sums in the same order as 1 without split, so the same C; the
+ * batch uses 1 instead); -1 = chosen per call (default: for multi-part
+ * algorithms pre-summed when its workspace fits in half of the free
+ * device memory, materialised too when that fits as well and a time model
+ * does not favour 1).  No counterpart in the reference, whose packing
+ * (plugin/ref_kernels/bli_packm_fmm_ref.c) and scatter
+ * (plugin/ref_kernels/bli_gemm_fmm_ref.c) are always fused. */
 int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
+/* The pack mode (0, 1, 2 as above) a column-major FP64 call of m x n x k
+ * with this algorithm and leaf would use now (the automatic choice depends
+ * on the free device memory); BF16x3 leaves stage their operand planes and
+ * report 1. */
+int  fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, long n, long k, int* pack);
 /* Split: 1 = each 128 x 128 tile of C is one workgroup looping over all R
  * products; 2, 4, 8 = that many workgroups per tile, each taking a
  * contiguous range of products (the classical algorithm, R = 1: a

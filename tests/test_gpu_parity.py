@@ -108,10 +108,11 @@ FMM_SHAPES = [(3, 13, 7), (31, 33, 35), (64, 64, 64), (97, 101, 103), (258, 260,
 
 @pytest.mark.parametrize("name", ALG_NAMES)
 @pytest.mark.parametrize("levels", [1, 2])
-@pytest.mark.parametrize("pack", [0, 1])
+@pytest.mark.parametrize("pack", [0, 1, 2])
 def test_fmm_exact_integers(name, levels, pack):
     """Bit-exact agreement with the oracle's FMM on integer inputs (uneven
-    splits included), in the fused and the pre-summed pack modes."""
+    splits included), in the fused, the pre-summed and the
+    products-materialised pack modes."""
     f = fmm(name)
     alg = F.Alg([f] * levels)
     alg.set_pack(pack)
@@ -129,7 +130,7 @@ def test_fmm_exact_integers(name, levels, pack):
 
 @pytest.mark.parametrize("name", ALG_NAMES)
 @pytest.mark.parametrize("levels", [1, 2])
-@pytest.mark.parametrize("pack", [0, 1])
+@pytest.mark.parametrize("pack", [0, 1, 2])
 def test_fmm_random_error(name, levels, pack):
     f = fmm(name)
     alg = F.Alg([f] * levels)
@@ -180,7 +181,7 @@ def test_operand_layouts(ta, tb, tc):
     rsB, csB = (n, 1) if tb else (1, k)
     rsC, csC = (n, 1) if tc else (1, m)
     alg = F.Alg([fmm("232")])
-    for pack in (0, 1):
+    for pack in (0, 1, 2):
         dA, dB, dC = put(A, ta), put(B, tb), put(C, tc)
         alg.set_pack(pack)
         F.check(F.lib().fmmhip_dgemm_fmm(alg.h, 0, m, n, k, 1.0, dA.ptr, rsA, csA, dB.ptr, rsB, csB, 1.0,

@@ -1,0 +1,147 @@
+"""GPU parity of the products-materialised pack mode (fmmhip_alg_set_pack 2):
+the MFMA kernel stores every product M_r once into a slot array and a W
+stage forms each C part as beta*C + sum_r alpha*W[c][r]*M_r, in ascending
+product order, the FMA chain of the fused epilogue.
+
+Bars (stated per test):
+  * integer-valued inputs: equal to the oracle's FMM bit for bit (every
+    algorithm and level, uneven shapes, row-major C, alpha/beta);
+  * uniform[-1,1] inputs: equal to the pre-summed mode (pack 1, no split)
+    bit for bit, for every group count of the product split, FP64, FP32
+    operands and the FP32 leaf -- the two modes run the same sums in the
+    same order;
+  * beta == 0 never reads C (NaN in C does not propagate).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fmm_blis_amd as F
+import oracle as O
+from conftest import ALG_NAMES, alg_path
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    rc = F.lib().fmmhip_init()
+    if rc != 0:
+        pytest.fail(f"no usable gfx950 device: {F.STATUS.get(rc)}")
+    yield
+
+
+def dev(X):
+    D = F.DeviceMatrix(X.shape[0], X.shape[1], dtype=X.dtype)
+    D.upload(X)
+    return D
+
+
+def alg(name, levels, pack, split=1):
+    a = F.Alg([F.new_fmm(alg_path(name))] * levels)
+    a.set_pack(pack)
+    a.set_split(split)
+    return a
+
+
+def run(a, alpha, A, B, beta, C, leaf=F.LEAF_NATIVE):
+    m, k = A.shape
+    n = B.shape[1]
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    if A.dtype == np.float32:
+        F.sgemm_fmm(a, m, n, k, alpha, dA, dB, beta, dC)
+    else:
+        F.dgemm_fmm(a, m, n, k, alpha, dA, dB, beta, dC, leaf=leaf)
+    F.sync()
+    return dC.to_numpy()
+
+
+def size_for(name, levels):
+    return {("444", 2): 150, ("333", 2): 200}.get((name, levels), 333)
+
+
+@pytest.mark.parametrize("name", ALG_NAMES)
+@pytest.mark.parametrize("levels", [1, 2])
+def test_wmat_equals_presum_bitwise(name, levels):
+    n0 = size_for(name, levels)
+    m, n, k = n0, n0 + 7, n0 - 5
+    A, B, C = O.uniform(m, k, 41, 0), O.uniform(k, n, 41, 1), O.uniform(m, n, 41, 2)
+    want = run(alg(name, levels, 1), 0.7, A, B, -1.3, C)
+    got = run(alg(name, levels, 2, -1), 0.7, A, B, -1.3, C)
+    say("wmat vs presum", name, levels, m, n, k)
+    assert np.array_equal(got, want)
+    ref = O.dgemm(0.7, A, B, -1.3, C.copy(order="F"))
+    assert O.normalized_error(got, ref, A, B) < 5e-15
+
+
+@pytest.mark.parametrize("groups", [1, 2, 4, 8])
+def test_wmat_groups_do_not_change_results(groups):
+    m = n = k = 520
+    A, B, C = O.uniform(m, k, 42, 0), O.uniform(k, n, 42, 1), O.uniform(m, n, 42, 2)
+    want = run(alg("222", 2, 1), 1.0, A, B, 1.0, C)
+    got = run(alg("222", 2, 2, groups), 1.0, A, B, 1.0, C)
+    assert np.array_equal(got, want), groups
+
+
+def test_wmat_exact_integers_and_row_major_c():
+    og = O.parse(alg_path("322"))
+    og2 = O.compose(og, og)
+    for (m, n, k) in [(97, 101, 103), (17, 1, 19), (300, 280, 36)]:
+        A, B, C = O.integers(m, k, 43, 0, 4), O.integers(k, n, 43, 1, 4), O.integers(m, n, 43, 2, 4)
+        for alpha, beta in [(1.0, 1.0), (2.0, -3.0), (0.5, 0.0)]:
+            want = O.fmm_dgemm(og2, alpha, A, B, beta, C.copy(order="F"))
+            say("wmat int", m, n, k, alpha, beta)
+            assert np.array_equal(run(alg("322", 2, 2, -1), alpha, A, B, beta, C), want)
+            # the same call with C stored by rows (slots stored by rows too)
+            a = alg("322", 2, 2, -1)
+            dA, dB, dC = dev(A), dev(B), dev(np.asfortranarray(C.T))
+            F.check(F.lib().fmmhip_dgemm_fmm(a.h, 0, m, n, k, alpha, dA.ptr, 1, m, dB.ptr, 1, k, beta, dC.ptr, n, 1,
+                                             None), "dgemm_fmm row-major C")
+            F.sync()
+            assert np.array_equal(dC.to_numpy().T, want)
+
+
+def test_wmat_beta_zero_ignores_nan():
+    m, n, k = 130, 90, 70
+    A, B = O.uniform(m, k, 44, 0), O.uniform(k, n, 44, 1)
+    C = np.full((m, n), np.nan, order="F")
+    got = run(alg("333", 1, 2, -1), 1.0, A, B, 0.0, C)
+    assert np.all(np.isfinite(got))
+    assert O.normalized_error(got, A @ B, A, B) < 1e-15
+
+
+def test_wmat_fp32_operands_and_fp32_leaf():
+    m = n = k = 300
+    A, B, C = (np.asfortranarray(O.uniform(d0, d1, 45, i).astype(np.float32))
+               for i, (d0, d1) in enumerate([(m, k), (k, n), (m, n)]))
+    want = run(alg("222", 2, 1), 1.0, A, B, 1.0, C)
+    got = run(alg("222", 2, 2, -1), 1.0, A, B, 1.0, C)
+    assert got.dtype == np.float32
+    assert np.array_equal(got, want)
+    A64, B64, C64 = (np.asfortranarray(X.astype(np.float64)) for X in (A, B, C))
+    want = run(alg("232", 1, 1), 1.0, A64, B64, 1.0, C64, leaf=F.LEAF_F32)
+    got = run(alg("232", 1, 2, -1), 1.0, A64, B64, 1.0, C64, leaf=F.LEAF_F32)
+    assert np.array_equal(got, want)
+
+
+def test_pack_for_reports_the_mode():
+    """fmmhip_alg_pack_for: the explicit mode as set (2 falls back to 1 for
+    the classical algorithm, which has one product), and the automatic one
+    within {0, 1, 2}: fused for the classical algorithm, pre-summed or
+    materialised for multi-part ones."""
+    a = F.Alg([F.strassen()] * 2)
+    for p in (0, 1, 2):
+        a.set_pack(p)
+        assert a.pack_for(1000, 900, 800) == p
+    a.set_pack(-1)
+    assert a.pack_for(4096, 4096, 4096) in (1, 2)
+    c = F.Alg(classical=True)
+    assert c.pack_for(4096, 4096, 4096) == 0
+    c.set_pack(2)
+    assert c.pack_for(4096, 4096, 4096) == 1

@@ -131,6 +131,18 @@ def test_alg_set_split_values():
     a.close()
 
 
+def test_alg_set_pack_values():
+    """Pack mode: -1 (per call), 0 fused, 1 pre-summed, 2 products
+    materialised; others rejected."""
+    a = F.Alg([F.strassen()] * 2)
+    for p in (-1, 0, 1, 2):
+        a.set_pack(p)
+    for p in (3, -2, 100):
+        with pytest.raises(F.FmmError):
+            a.set_pack(p)
+    a.close()
+
+
 def test_alg_create_rejects_invalid_table(tmp_path):
     text = open(alg_path("222")).read().replace("1 0 1 0 1 -1 0", "1 0 1 0 1 1 0", 1)
     p = tmp_path / "bad222.txt"
