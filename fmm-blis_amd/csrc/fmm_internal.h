@@ -42,6 +42,10 @@ struct fmmhip_alg_s {
     int  pack;                  /* 0 fused, 1 pre-summed workspace, -1 = chosen per call */
     int  keep_all;              /* never evict cached descriptors (owners that capture graphs) */
     int  split;                 /* product groups per tile: -1 chosen per call, 1 (none), 2, 4, 8 */
+    /* two-level algorithms: the two levels' own tables (level 0 outer),
+     * for the two-stage operand summation; NULL otherwise */
+    int  lv_m[2], lv_k[2], lv_n[2], lv_R[2];
+    int *lv_U[2], *lv_V[2];     /* dense [part][r] */
     void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
 };
 
@@ -120,6 +124,23 @@ typedef struct fmmhip_wsum_s {
     const fmm_entry_t* terms;
 } fmmhip_wsum_t;
 int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
+
+/* One segment of a stage of the two-stage operand summation
+ * (fmmhip_presum.hip): R sums of np parts, every sum a full leaf.  Parts
+ * lie at `parts[p].off` from the stage's input, sum r is written at
+ * s_off + r*pstride from the stage's output. */
+typedef struct fmmhip_stage_seg_s {
+    long s_off;
+    long pstride;
+    int R, np;                  /* np <= 16 */
+    const int* ptr;             /* device CSR over the R sums into ent (absolute indices) */
+    const fmm_entry_t* ent;     /* part = index into parts, coef */
+    const fmm_entry_t* parts;   /* off, rows, cols */
+} fmmhip_stage_seg_t;
+/* One stage: every segment over the leaf layout (Lc, Ls, ldo, swap as in
+ * fmmhip_presum_seg_t), input X with leading dimension ld, output S. */
+int fmmhip_presum_stage(int is_double, const void* X, long ld, void* S, int swap, int Lc, int Ls, int ldo,
+                        const fmmhip_stage_seg_t* segs, int nseg, void* stream);
 
 /* Summation stage of the pre-summed pack mode (fmmhip_presum.hip). */
 int fmmhip_presum_d(const double* X, long ld, const int* ptr, const fmm_entry_t* ent, int R, int swap,

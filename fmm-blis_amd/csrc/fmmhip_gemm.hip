@@ -11,6 +11,7 @@
 #include <stdlib.h>
 
 #include "fmmhip_kernel.h"
+#include "fmmhip_host.h"
 
 using fmmk::KArgs;
 using fmmk::BM;
@@ -38,6 +39,10 @@ struct DevDesc {
     bool has_presum = false;
     DevList mat;                 // as presum, W = one full-leaf slot per product, then the W-stage tables
     bool has_mat = false;
+    void* kron_dev = nullptr;    // two-stage summation tables (one allocation)
+    std::vector<char> kron_host; // their host image (source of the upload)
+    KronView kron{};
+    bool has_kron = false;
     unsigned long last_use = 0;
 };
 
@@ -68,6 +73,7 @@ void free_desc(DevDesc* d)
     free_list(d->fused);
     free_list(d->presum);
     free_list(d->mat);
+    if (d->kron_dev) (void)hipFree(d->kron_dev);
     delete d;
 }
 
@@ -235,6 +241,183 @@ int build_mat_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const De
 
 // Returns the cached descriptor for geometry g (building and uploading the
 // fused list, and the pre-summed list when asked, stream-ordered).
+// Whether an operand's sums take the two-stage form: two-level algorithms
+// whose composed part count exceeds what the one-pass part-staged kernel
+// runs at full occupancy (64 parts; its LDS holds every part of a column
+// of elements), with at most 16 parts per level.
+constexpr int kKronMinParts = 65;
+
+bool kron_operand(const struct fmmhip_alg_s* alg, bool is_a)
+{
+    if (alg->levels != 2 || !alg->lv_U[0] || !alg->lv_U[1]) return false;
+    const int np = is_a ? alg->mt * alg->kt : alg->kt * alg->nt;
+    const int np0 = is_a ? alg->lv_m[0] * alg->lv_k[0] : alg->lv_k[0] * alg->lv_n[0];
+    const int np1 = is_a ? alg->lv_m[1] * alg->lv_k[1] : alg->lv_k[1] * alg->lv_n[1];
+    return np >= kKronMinParts && np0 <= FMMHIP_BATCH_PARTS_MAX && np1 <= FMMHIP_BATCH_PARTS_MAX;
+}
+
+// Host image of one operand's two-stage tables.  Composed part (row
+// a0*rows1 + a1, col b0*cols1 + b1) with coefficient X0[(a0,b0)][r0] *
+// X1[(a1,b1)][r1] for product r0*R1 + r1 (the composition of
+// fmm_spec.c).  Stage 1, segment per outer block b = (a0,b0): the inner
+// sums Y[b][r1] = sum_(a1,b1) X1[(a1,b1)][r1] part(b; a1,b1), written as
+// leaf b*R1 + r1 of the stage-1 region.  Stage 2, segment per inner
+// product r1: S[r0*R1 + r1] = sum_b X0[b][r0] Y[b][r1].
+struct KronImage {
+    std::vector<int32_t> ptr0, ptr1;
+    std::vector<fmm_entry_t> ent0, ent1, parts1, parts2;
+    std::vector<fmmhip_stage_seg_t> seg1, seg2;
+    long x1_elems = 0;
+};
+
+void kron_image(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool is_a, const WsLayout& wl, KronImage& k)
+{
+    const int rows0 = is_a ? alg->lv_m[0] : alg->lv_k[0], cols0 = is_a ? alg->lv_k[0] : alg->lv_n[0];
+    const int rows1 = is_a ? alg->lv_m[1] : alg->lv_k[1], cols1 = is_a ? alg->lv_k[1] : alg->lv_n[1];
+    const int R0 = alg->lv_R[0], R1 = alg->lv_R[1];
+    const int* X0 = is_a ? alg->lv_U[0] : alg->lv_V[0];
+    const int* X1 = is_a ? alg->lv_U[1] : alg->lv_V[1];
+    const int np0 = rows0 * cols0, np1 = rows1 * cols1;
+    // CSR over products of one level, parts numbered row*cols + col
+    auto csr = [](const int* X, int np, int R, std::vector<int32_t>& ptr, std::vector<fmm_entry_t>& ent) {
+        ptr.assign((size_t)R + 1, 0);
+        for (int r = 0; r < R; r++) {
+            ptr[r] = (int32_t)ent.size();
+            for (int p = 0; p < np; p++)
+                if (X[(size_t)p * R + r]) {
+                    fmm_entry_t e;
+                    memset(&e, 0, sizeof(e));
+                    e.coef = (double)X[(size_t)p * R + r];
+                    e.part = p;
+                    ent.push_back(e);
+                }
+        }
+        ptr[R] = (int32_t)ent.size();
+    };
+    csr(X0, np0, R0, k.ptr0, k.ent0);
+    csr(X1, np1, R1, k.ptr1, k.ent1);
+    // composed part geometry: the operand's rows and columns split directly
+    const long wr = is_a ? g.m : g.k, wc = is_a ? g.k : g.n;
+    const int tr = rows0 * rows1, tc = cols0 * cols1;
+    std::vector<long> szr(tr), ofr(tr), szc(tc), ofc(tc);
+    fmm_part_split(wr, tr, szr.data(), ofr.data());
+    fmm_part_split(wc, tc, szc.data(), ofc.data());
+    const long rs = is_a ? g.rsA : g.rsB, cs = is_a ? g.csA : g.csB;
+    const long plane = (long)wl.ldo * wl.Ls;
+    const long lr = is_a ? g.lm : g.lk, lc = is_a ? g.lk : g.ln;   // leaf rows and columns
+    for (int a0 = 0; a0 < rows0; a0++)
+        for (int b0 = 0; b0 < cols0; b0++)
+            for (int a1 = 0; a1 < rows1; a1++)
+                for (int b1 = 0; b1 < cols1; b1++) {
+                    const int I = a0 * rows1 + a1, P = b0 * cols1 + b1;
+                    fmm_entry_t e;
+                    memset(&e, 0, sizeof(e));
+                    e.off = ofr[I] * rs + ofc[P] * cs;
+                    e.rows = (int32_t)szr[I];
+                    e.cols = (int32_t)szc[P];
+                    k.parts1.push_back(e);           // block (a0,b0), part a1*cols1 + b1
+                }
+    for (int r1 = 0; r1 < R1; r1++)
+        for (int b = 0; b < np0; b++) {
+            fmm_entry_t e;
+            memset(&e, 0, sizeof(e));
+            e.off = ((long)b * R1 + r1) * plane;
+            e.rows = (int32_t)lr;
+            e.cols = (int32_t)lc;
+            k.parts2.push_back(e);
+        }
+    for (int b = 0; b < np0; b++) {
+        fmmhip_stage_seg_t sg;
+        memset(&sg, 0, sizeof(sg));
+        sg.s_off = (long)b * R1 * plane;
+        sg.pstride = plane;
+        sg.R = R1;
+        sg.np = np1;
+        k.seg1.push_back(sg);
+    }
+    for (int r1 = 0; r1 < R1; r1++) {
+        fmmhip_stage_seg_t sg;
+        memset(&sg, 0, sizeof(sg));
+        sg.s_off = (long)r1 * plane;
+        sg.pstride = (long)R1 * plane;
+        sg.R = R0;
+        sg.np = np0;
+        k.seg2.push_back(sg);
+    }
+    k.x1_elems = (long)np0 * R1 * plane;
+}
+
+// Builds and uploads the two-stage tables of the operands that take them.
+int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, bool b_mn, DevDesc* d,
+               hipStream_t stream)
+{
+    KronImage im[2];
+    const bool use[2] = {kron_operand(alg, true), kron_operand(alg, false)};
+    const WsLayout wl[2] = {ws_layout_a(g, a_mn), ws_layout_b(g, b_mn)};
+    for (int o = 0; o < 2; o++)
+        if (use[o]) kron_image(alg, g, o == 0, wl[o], im[o]);
+    // layout of the one allocation: per operand ptr0, ptr1, ent0, ent1, parts1, parts2, seg1, seg2
+    std::vector<char>& h = d->kron_host;
+    h.clear();
+    auto place = [&](const void* src, size_t bytes) {
+        const size_t off = (h.size() + 255) & ~(size_t)255;
+        h.resize(off + bytes);
+        if (bytes) memcpy(h.data() + off, src, bytes);
+        return off;
+    };
+    size_t off[2][8] = {};
+    for (int o = 0; o < 2; o++) {
+        if (!use[o]) continue;
+        KronImage& k = im[o];
+        off[o][0] = place(k.ptr0.data(), k.ptr0.size() * sizeof(int32_t));
+        off[o][1] = place(k.ptr1.data(), k.ptr1.size() * sizeof(int32_t));
+        off[o][2] = place(k.ent0.data(), k.ent0.size() * sizeof(fmm_entry_t));
+        off[o][3] = place(k.ent1.data(), k.ent1.size() * sizeof(fmm_entry_t));
+        off[o][4] = place(k.parts1.data(), k.parts1.size() * sizeof(fmm_entry_t));
+        off[o][5] = place(k.parts2.data(), k.parts2.size() * sizeof(fmm_entry_t));
+        off[o][6] = place(k.seg1.data(), k.seg1.size() * sizeof(fmmhip_stage_seg_t));
+        off[o][7] = place(k.seg2.data(), k.seg2.size() * sizeof(fmmhip_stage_seg_t));
+    }
+    if (h.empty()) return FMMHIP_SUCCESS;
+    if (hipMalloc(&d->kron_dev, h.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        d->kron_dev = nullptr;
+        return FMMHIP_ERR_NOMEM;
+    }
+    char* dev = (char*)d->kron_dev;
+    KronView& v = d->kron;
+    memset(&v, 0, sizeof(v));
+    for (int o = 0; o < 2; o++) {
+        if (!use[o]) continue;
+        KronImage& k = im[o];
+        fmmhip_stage_seg_t* s1 = (fmmhip_stage_seg_t*)(h.data() + off[o][6]);
+        fmmhip_stage_seg_t* s2 = (fmmhip_stage_seg_t*)(h.data() + off[o][7]);
+        const int np1 = (int)(k.parts1.size() / k.seg1.size());
+        const int np0 = (int)k.seg1.size();
+        for (size_t b = 0; b < k.seg1.size(); b++) {
+            s1[b].ptr = (const int*)(dev + off[o][1]);
+            s1[b].ent = (const fmm_entry_t*)(dev + off[o][3]);
+            s1[b].parts = (const fmm_entry_t*)(dev + off[o][4]) + b * np1;
+        }
+        for (size_t r = 0; r < k.seg2.size(); r++) {
+            s2[r].ptr = (const int*)(dev + off[o][0]);
+            s2[r].ent = (const fmm_entry_t*)(dev + off[o][2]);
+            s2[r].parts = (const fmm_entry_t*)(dev + off[o][5]) + r * np0;
+        }
+        const fmmhip_stage_seg_t* d1 = (const fmmhip_stage_seg_t*)(dev + off[o][6]);
+        const fmmhip_stage_seg_t* d2 = (const fmmhip_stage_seg_t*)(dev + off[o][7]);
+        if (o == 0) {
+            v.useA = true; v.a1 = d1; v.a2 = d2; v.na1 = (int)k.seg1.size(); v.na2 = (int)k.seg2.size();
+        } else {
+            v.useB = true; v.b1 = d1; v.b2 = d2; v.nb1 = (int)k.seg1.size(); v.nb2 = (int)k.seg2.size();
+        }
+        v.x1_elems = std::max(v.x1_elems, k.x1_elems);
+    }
+    if (hipMemcpyAsync(d->kron_dev, h.data(), h.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return FMMHIP_ERR_HIP;
+    return FMMHIP_SUCCESS;
+}
+
 int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool want_mat, bool a_mn, bool b_mn,
              hipStream_t stream, DevDesc** out)
 {
@@ -271,6 +454,16 @@ int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool want_
         if (!rc) rc = upload_list(d->presum, alg->R, stream);
         if (rc) { free_list(d->presum); return rc; }
         d->has_presum = true;
+    }
+    if (want_presum && !d->has_kron && (kron_operand(alg, true) || kron_operand(alg, false))) {
+        int rc = build_kron(alg, g, a_mn, b_mn, d, stream);
+        if (rc) {
+            if (d->kron_dev) (void)hipFree(d->kron_dev);
+            d->kron_dev = nullptr;
+            memset(&d->kron, 0, sizeof(d->kron));
+            return rc;
+        }
+        d->has_kron = true;
     }
     if (want_mat && !d->has_mat) {
         int rc = build_mat_list(alg, g, d->fused, d->presum, d->mat);
@@ -313,9 +506,6 @@ void* workspace_get(hipStream_t st, size_t bytes)
     return p;
 }
 
-}  // namespace
-#include "fmmhip_host.h"
-namespace {
 
 // Pairs along the contiguous dimension can be loaded as one 16-byte vector
 // when the base, the leading dimension and every part offset keep them
@@ -534,6 +724,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
                 dv.mat_ptr = d->mat.dptr();
                 dv.mat_ent = d->mat.dent();
             }
+            if (presum) dv.kron = d->kron;
         }
     }
     if (rc) return rc;
@@ -574,6 +765,12 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.ent = dv.presum_ent;
         ka.va = ka.vb = 1;           // workspace leaves are 16-byte aligned with even ldo
         if (shape < 0) shape = 0;    // one part per product
+        if (dv.kron.useA || dv.kron.useB) {
+            // stage-1 leaves of the two-stage summation, after the B leaves
+            P->kron = dv.kron;
+            P->x1_off = (P->ws_bytes + 255) & ~(size_t)255;
+            P->ws_bytes = P->x1_off + esz * (size_t)dv.kron.x1_elems;
+        }
     } else {
         ka.A = A;
         ka.B = B;
@@ -664,6 +861,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
 void bind_workspace(Prepared* P, void* ws)
 {
     char* base = (char*)ws;
+    if (P->kron.useA || P->kron.useB) P->x1 = base + P->x1_off;
     if (P->wmat) {
         P->ka.C = base + P->m_off;
         P->ka.tmp = P->ka.C;
@@ -702,23 +900,26 @@ int launch_prepared(const Prepared* P)
         rc = fmmhip_split_zero(dbl, P->ka.tmp, P->ka.tmp_elems, P->ka.groups, st);
         if (rc) return rc;
     }
-    if (P->presum && P->segA.parts && P->segA.np <= kPartStagedMax && P->segB.np <= kPartStagedMax) {
-        rc = fmmhip_presum_seg(dbl, &P->segA, st);
-        if (!rc) rc = fmmhip_presum_seg(dbl, &P->segB, st);
-        if (rc) return rc;
-    } else if (P->presum) {
-        const fmmhip_presum_seg_t& a = P->segA;
-        const fmmhip_presum_seg_t& b = P->segB;
-        if (P->kind == Kind::S32) {
-            rc = fmmhip_presum_s((const float*)a.X, a.ld, a.ptr, a.ent, a.R, a.swap, a.Lc, a.Ls, a.ldo, (float*)a.S, st);
-            if (!rc) rc = fmmhip_presum_s((const float*)b.X, b.ld, b.ptr, b.ent, b.R, b.swap, b.Lc, b.Ls, b.ldo,
-                                          (float*)b.S, st);
-        } else {
-            rc = fmmhip_presum_d((const double*)a.X, a.ld, a.ptr, a.ent, a.R, a.swap, a.Lc, a.Ls, a.ldo, (double*)a.S,
-                                 st);
-            if (!rc) rc = fmmhip_presum_d((const double*)b.X, b.ld, b.ptr, b.ent, b.R, b.swap, b.Lc, b.Ls, b.ldo,
-                                          (double*)b.S, st);
-        }
+    if (P->presum) {
+        // per operand: the two-stage form, the part-staged one-pass form, or
+        // the entry-driven one (part counts beyond the part-staged kernel's)
+        auto sum_operand = [&](const fmmhip_presum_seg_t& sg, bool two_stage, const fmmhip_stage_seg_t* s1, int n1,
+                               const fmmhip_stage_seg_t* s2, int n2) -> int {
+            if (two_stage) {
+                int r = fmmhip_presum_stage(dbl, sg.X, sg.ld, P->x1, sg.swap, sg.Lc, sg.Ls, sg.ldo, s1, n1, st);
+                if (!r) r = fmmhip_presum_stage(dbl, P->x1, sg.ldo, sg.S, sg.swap, sg.Lc, sg.Ls, sg.ldo, s2, n2, st);
+                return r;
+            }
+            if (sg.parts && sg.np <= kPartStagedMax) return fmmhip_presum_seg(dbl, &sg, st);
+            if (P->kind == Kind::S32)
+                return fmmhip_presum_s((const float*)sg.X, sg.ld, sg.ptr, sg.ent, sg.R, sg.swap, sg.Lc, sg.Ls, sg.ldo,
+                                       (float*)sg.S, st);
+            return fmmhip_presum_d((const double*)sg.X, sg.ld, sg.ptr, sg.ent, sg.R, sg.swap, sg.Lc, sg.Ls, sg.ldo,
+                                   (double*)sg.S, st);
+        };
+        const KronView& kv = P->kron;
+        rc = sum_operand(P->segA, kv.useA, kv.a1, kv.na1, kv.a2, kv.na2);
+        if (!rc) rc = sum_operand(P->segB, kv.useB, kv.b1, kv.nb1, kv.b2, kv.nb2);
         if (rc) return rc;
     }
     if (timed) tm_record(&tc.e1, st);

@@ -11,6 +11,16 @@
 
 enum class Kind { D64, D_F32LEAF, S32 };
 
+// Two-stage summation of a two-level algorithm's operands (device tables):
+// stage 1 forms, per outer block, the inner level's sums of the block's
+// parts; stage 2 forms, per inner product, the outer level's sums of those.
+struct KronView {
+    bool useA, useB;
+    const fmmhip_stage_seg_t *a1, *a2, *b1, *b2;   // stage 1 and 2 segments per operand
+    int na1, na2, nb1, nb2;
+    long x1_elems;               // stage-1 leaves (the larger operand's)
+};
+
 struct Prepared {
     Kind kind;
     hipStream_t stream;
@@ -27,6 +37,9 @@ struct Prepared {
     size_t ws_bytes;             // workspace: summation leaves, then the split temporaries
     fmmhip_presum_seg_t segA, segB;
     size_t tmp_off;              // split temporaries: workspace offset (ka.groups > 1)
+    KronView kron;               // two-stage summation (pre-summed modes, two-level algorithms)
+    size_t x1_off;               // its stage-1 leaves: workspace offset
+    void* x1;
     bool wmat;                   // products materialised: kernel into slots, then the W stage
     size_t m_off;                // slots: workspace offset
     fmmhip_wsum_t wsum;          // the W stage's arguments (slots bound with the workspace)
@@ -43,6 +56,7 @@ struct DescView {
     const fmm_entry_t* presum_ent;
     const int* mat_ptr;          // products-materialised list (one-problem path only)
     const fmm_entry_t* mat_ent;
+    KronView kron;               // two-stage summation tables (one-problem path only)
 };
 
 // Where prepare_call takes the lists from; without one, from the

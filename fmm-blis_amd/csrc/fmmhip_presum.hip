@@ -198,6 +198,31 @@ __global__ __launch_bounds__(TB) void presum_parts_batch_kernel(const fmmhip_pre
     }
 }
 
+// One stage of the two-stage summation of a two-level algorithm: segment
+// si = blockIdx.x / bps forms its R sums of up to NPMAX parts over the
+// whole leaf (units of TB elements along c of one s, grid-strided over the
+// segment's bps blocks) with the part-staged unit above.
+template <typename T, int NPMAX, int TB>
+__global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ X, long ld, T* __restrict__ S,
+                                                          int swap, int Lc, int Ls, int ldo,
+                                                          const fmmhip_stage_seg_t* __restrict__ segs, int bps)
+{
+    const int si = (int)blockIdx.x / bps;
+    const fmmhip_stage_seg_t sg = segs[si];
+    __shared__ double xs[NPMAX][TB];
+    __shared__ long p_off[NPMAX];
+    __shared__ int p_c[NPMAX], p_s[NPMAX];
+    parts_meta<NPMAX, TB>(sg.parts, sg.np, swap, p_off, p_c, p_s);
+    __syncthreads();
+    const long nbc = (Lc + TB - 1) / TB;
+    for (long u = (long)blockIdx.x - (long)si * bps; u < nbc * Ls; u += bps) {
+        const int s = (int)(u / nbc);
+        const int c = (int)(u - (long)s * nbc) * TB + threadIdx.x;
+        parts_unit<T, NPMAX, TB>(X, ld, sg.ptr, sg.ent, sg.np, sg.R, Lc, ldo, sg.pstride, S + sg.s_off, s, c, xs,
+                                 p_off, p_c, p_s);
+    }
+}
+
 template <typename T, int NPMAX, int TB>
 int launch_parts(const fmmhip_presum_seg_t* g, hipStream_t st)
 {
@@ -228,6 +253,27 @@ extern "C" int fmmhip_presum_seg(int is_double, const fmmhip_presum_seg_t* seg, 
     if (!seg || !seg->parts || seg->np <= 0) return FMMHIP_ERR_INVALID;
     return is_double ? launch_parts_typed<double>(seg, (hipStream_t)stream)
                      : launch_parts_typed<float>(seg, (hipStream_t)stream);
+}
+
+extern "C" int fmmhip_presum_stage(int is_double, const void* X, long ld, void* S, int swap, int Lc, int Ls, int ldo,
+                                   const fmmhip_stage_seg_t* segs, int nseg, void* stream)
+{
+    if (nseg <= 0 || Lc <= 0 || Ls <= 0) return FMMHIP_SUCCESS;
+    constexpr int TB = 256;
+    // about 8 blocks per CU in all, at most one per unit of the segment
+    const long units = (long)((Lc + TB - 1) / TB) * Ls;
+    const long want = (2048 + nseg - 1) / nseg;
+    const int bps = (int)(units < want ? units : want);
+    if ((long)bps * nseg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
+    if (is_double)
+        hipLaunchKernelGGL((presum_stage_kernel<double, FMMHIP_BATCH_PARTS_MAX, TB>), dim3((unsigned)(bps * nseg)),
+                           dim3(TB), 0, (hipStream_t)stream, (const double*)X, ld, (double*)S, swap, Lc, Ls, ldo,
+                           segs, bps);
+    else
+        hipLaunchKernelGGL((presum_stage_kernel<float, FMMHIP_BATCH_PARTS_MAX, TB>), dim3((unsigned)(bps * nseg)),
+                           dim3(TB), 0, (hipStream_t)stream, (const float*)X, ld, (float*)S, swap, Lc, Ls, ldo, segs,
+                           bps);
+    return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
 }
 
 extern "C" int fmmhip_presum_batch(int is_double, const fmmhip_presum_seg_t* segs, int nseg, const int* boff,

@@ -417,3 +417,31 @@ def test_concurrent_host_threads_on_one_stream():
         og_used = og2 if A.shape[0] > 200 else og
         want = O.fmm_dgemm(og_used, 1.0, A, B, 1.0, C.copy(order="F"))
         assert np.array_equal(dC.to_numpy(), want), (t, A.shape)
+
+
+@pytest.mark.parametrize("name", ["333", "444"])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, True), (True, False)])
+def test_two_stage_sums_exact_any_layout(name, ta, tb):
+    """Two-level 333 and 444 (81 and 256 composed parts per operand) form
+    their operand sums in two stages (inner level per outer block, then the
+    outer level): exact against the oracle's FMM on integer inputs, with
+    row- or column-major operands and uneven parts, in both pre-summed
+    modes."""
+    og = O.parse(alg_path(name))
+    og = O.compose(og, og)
+    m, n, k = (163, 151, 145) if name == "333" else (150, 133, 141)
+    A, B, C = O.integers(m, k, 9, 0, 3), O.integers(k, n, 9, 1, 3), O.integers(m, n, 9, 2, 3)
+    want = O.fmm_dgemm(og, 1.0, A, B, -2.0, C.copy(order="F"))
+    rsA, csA = (k, 1) if ta else (1, m)
+    rsB, csB = (n, 1) if tb else (1, k)
+    for pack in (1, 2):
+        alg = F.Alg([fmm(name)] * 2)
+        alg.set_pack(pack)
+        dA = dev(np.asfortranarray(A.T)) if ta else dev(A)
+        dB = dev(np.asfortranarray(B.T)) if tb else dev(B)
+        dC = dev(C)
+        say("two-stage", name, ta, tb, pack)
+        F.check(F.lib().fmmhip_dgemm_fmm(alg.h, 0, m, n, k, 1.0, dA.ptr, rsA, csA, dB.ptr, rsB, csB, -2.0,
+                                         dC.ptr, 1, m, None), "dgemm_fmm two-stage")
+        F.sync()
+        assert np.array_equal(dC.to_numpy(), want), pack
