@@ -481,13 +481,29 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t oPB = al(planesA), oEA = oPB + al(planesB), oEB = oEA + al(expA);
     const long tiles = (LpA / Cfg3::BM) * (LpB / Cfg3::BN);
-    // product split (bf16 MFMA rate: 8 or 6 plane products per leaf product)
+    // products materialised (pack 2, or automatic when the slots fit in half
+    // of the free device memory): the kernel stores each product once and
+    // the W stage applies W -- the kernel's W epilogue then only stores
+    bool wmat = R > 1 && (alg->pack == 2 || alg->pack < 0);
+    if (wmat && alg->pack < 0) {
+        size_t fr = 0, tot = 0;
+        wmat = hipMemGetInfo(&fr, &tot) == hipSuccess && 8.0 * R * (double)g.lm * (double)g.ln < 0.5 * (double)fr;
+    }
+    MatView mv;
+    memset(&mv, 0, sizeof(mv));
+    if (wmat && (rc = mat_view(alg, g, st, &mv))) return rc;
+    int cus = 256;
+    fmmhip_device_cu_count(&cus);
+    // product split (bf16 MFMA rate: 8 or 6 plane products per leaf product;
+    // one workgroup per CU, LDS); materialised, any count up to R, no copies
     const double c_span = (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0;   // the temporaries' extent
-    const int groups = split_groups(alg->split, R, tiles, 2.0 * terms * R * (double)g.lm * (double)g.lk * (double)g.ln,
-                                    8.0 * c_span, false, 2.5e15, 1);   // one workgroup per CU (LDS)
-    const long tmp_elems = groups > 1 ? (((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L) : 0;
-    const size_t oT = oEB + al(expB);
-    const size_t bytes = oT + (size_t)groups * tmp_elems * 8;
+    const int groups = wmat ? (alg->split > 0 ? std::min(alg->split, R) : mat_groups(R, tiles, cus))
+                            : split_groups(alg->split, R, tiles,
+                                           2.0 * terms * R * (double)g.lm * (double)g.lk * (double)g.ln, 8.0 * c_span,
+                                           false, 2.5e15, 1);
+    const long tmp_elems = (!wmat && groups > 1) ? (((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L) : 0;
+    const size_t oT = oEB + al(expB);        // split temporaries, or the product slots
+    const size_t bytes = oT + (wmat ? (size_t)R * (size_t)mv.slot * 8 : (size_t)groups * tmp_elems * 8);
     std::unique_lock<std::mutex> lk = enqueue_lock();   // held until the last launch below
     char* ws = (char*)workspace_for(st, bytes);
     if (!ws) return FMMHIP_ERR_NOMEM;
@@ -517,7 +533,7 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     if (csB == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gB, blk, 0, st, sb);
     else hipLaunchKernelGGL(bf3::split_kernel<false>, gB, blk, 0, st, sb);
     if (hipGetLastError() != hipSuccess) return FMMHIP_ERR_HIP;
-    if (groups > 1 && (rc = fmmhip_split_zero(1, ws + oT, tmp_elems, groups, st))) return rc;
+    if (!wmat && groups > 1 && (rc = fmmhip_split_zero(1, ws + oT, tmp_elems, groups, st))) return rc;
     if (timed) timing_record(&e1, st);
 
     bf3::GemmArgs ga;
@@ -540,6 +556,21 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     ga.gbit = groups == 8 ? 3 : groups == 4 ? 2 : groups == 2 ? 1 : 0;
     ga.tmp = groups > 1 ? (double*)(ws + oT) : nullptr;
     ga.tmp_elems = tmp_elems;
+    if (wmat) {
+        // slot r of lm x ln per product: one entry each, coefficient 1, first writer
+        ga.wptr = mv.wptr;
+        ga.ent = mv.ent;
+        ga.C = (double*)(ws + oT);
+        ga.rsC = mv.swap ? mv.ldm : 1;
+        ga.csC = mv.swap ? 1 : mv.ldm;
+        ga.alpha = 1.0;
+        ga.beta = 0.0;
+        ga.min_cm = (int)g.lm;
+        ga.min_cn = (int)g.ln;
+        ga.gbit = 0;
+        ga.tmp = ga.C;           // every group's Cb: the slots
+        ga.tmp_elems = 0;
+    }
     const long nwg = (long)ga.tiles_m * ga.tiles_n * groups;
     if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
     if (terms == 8)
@@ -552,6 +583,24 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     if (timed) {
         timing_record(&e2, st);
         timing_push(e0, e1, e2, true);
+    }
+    if (wmat) {
+        fmmhip_wsum_t w;
+        memset(&w, 0, sizeof(w));
+        w.M = ws + oT;
+        w.ldm = mv.ldm;
+        w.swap = mv.swap;
+        w.Lc = mv.Lc;
+        w.Ls = mv.Ls;
+        w.C = C;
+        w.rsC = rsC;
+        w.csC = csC;
+        w.alpha = alpha;
+        w.beta = beta;
+        w.np = mv.np;
+        w.parts = mv.parts;
+        w.terms = mv.terms;
+        return fmmhip_wsum(1, &w, st);
     }
     if (groups > 1) return fmmhip_split_reduce(1, C, m, n, rsC, csC, beta, ws + oT, tmp_elems, groups, st);
     return FMMHIP_SUCCESS;

@@ -983,6 +983,26 @@ int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int*
     return FMMHIP_SUCCESS;
 }
 
+int mat_view(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, MatView* v)
+{
+    DevDesc* d = nullptr;
+    int rc = get_desc(alg, g, true, true, g.rsA == 1, g.rsB != 1, st, &d);
+    if (rc) return rc;
+    const int R = alg->R;
+    const SlotLayout sl = slot_layout(g);
+    v->wptr = d->mat.dptr() + 2 * (R + 1);
+    v->ent = d->mat.dent();
+    v->np = alg->mt * alg->nt;
+    v->parts = d->mat.dent() + 3 * R;
+    v->terms = v->parts + v->np;
+    v->swap = sl.swap;
+    v->Lc = sl.Lc;
+    v->Ls = sl.Ls;
+    v->ldm = sl.ldm;
+    v->slot = sl.slot;
+    return FMMHIP_SUCCESS;
+}
+
 void* workspace_for(hipStream_t st, size_t bytes) { return workspace_get(st, bytes); }
 
 std::mutex g_enqueue_mu;
@@ -1093,7 +1113,17 @@ extern "C" int fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m,
     if (!alg || !pack || m < 0 || n < 0 || k < 0) return FMMHIP_ERR_INVALID;
     int rc = fmmhip_init();
     if (rc) return rc;
-    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6) { *pack = 1; return FMMHIP_SUCCESS; }
+    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6) {
+        // planes always staged; products materialised as bf16x3_gemm decides
+        bool wmat = alg->R > 1 && (alg->pack == 2 || alg->pack < 0);
+        if (wmat && alg->pack < 0) {
+            size_t fr = 0, tot = 0;
+            const double lm = (double)((m + alg->mt - 1) / alg->mt), ln = (double)((n + alg->nt - 1) / alg->nt);
+            wmat = hipMemGetInfo(&fr, &tot) == hipSuccess && 8.0 * alg->R * lm * ln < 0.5 * (double)fr;
+        }
+        *pack = wmat ? 2 : 1;
+        return FMMHIP_SUCCESS;
+    }
     const Kind kind = leaf == FMMHIP_LEAF_F32 ? Kind::D_F32LEAF : Kind::D64;
     fmm_geom_t g;
     memset(&g, 0, sizeof(g));

@@ -96,6 +96,22 @@ fmmhip_alg_t classical_alg();
 // The cached fused descriptor of geometry g (device CSR U|V|W and entries).
 int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int** dptr,
                const fmm_entry_t** dent);
+// The products-materialised lists of geometry g (cached like the others):
+// the kernel's W list (one slot entry per product) and the W stage's part
+// table and terms, with the slot layout.
+struct MatView {
+    const int* wptr;             // CSR over products into ent
+    const fmm_entry_t* ent;
+    const fmm_entry_t* parts;
+    const fmm_entry_t* terms;
+    int np;
+    int swap, Lc, Ls;
+    long ldm, slot;
+};
+int mat_view(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, MatView* v);
+// Groups per tile of the products-materialised mode for a grid of `tiles`
+// workgroups on `slots` resident workgroup slots.
+int mat_groups(int R, long tiles, int slots);
 // The stream's grow-only workspace of at least `bytes` (nullptr: no memory).
 void* workspace_for(hipStream_t st, size_t bytes);
 // Held by a call from its workspace lookup to its last launch (host-side

@@ -239,8 +239,9 @@ int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
 int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
 /* The pack mode (0, 1, 2 as above) a column-major FP64 call of m x n x k
  * with this algorithm and leaf would use now (the automatic choice depends
- * on the free device memory); BF16x3 leaves stage their operand planes and
- * report 1. */
+ * on the free device memory).  BF16x3 leaves always stage their operand
+ * planes; they report 2 when the products are materialised (pack 2, or -1
+ * with several products and the slots in half of the free memory), else 1. */
 int  fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, long n, long k, int* pack);
 /* Split: 1 = each 128 x 128 tile of C is one workgroup looping over all R
  * products; 2, 4, 8 = that many workgroups per tile, each taking a

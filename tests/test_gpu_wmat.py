@@ -145,3 +145,22 @@ def test_pack_for_reports_the_mode():
     assert c.pack_for(4096, 4096, 4096) == 0
     c.set_pack(2)
     assert c.pack_for(4096, 4096, 4096) == 1
+
+
+@pytest.mark.parametrize("name,levels", [("222", 1), ("222", 2), ("333", 1)])
+@pytest.mark.parametrize("leaf", [F.LEAF_BF16X3, F.LEAF_BF16X3_6])
+def test_bf16x3_materialised_equals_in_kernel_epilogue(name, levels, leaf):
+    """BF16x3 leaf: with the products materialised (pack 2, and -1 by
+    default) the kernel only stores each product and the W stage applies W;
+    the result equals the in-kernel W epilogue (pack 1, no split) bit for
+    bit, whatever the group count."""
+    m, n, k = 410, 397, 388
+    A, B, C = O.uniform(m, k, 46, 0), O.uniform(k, n, 46, 1), O.uniform(m, n, 46, 2)
+    want = run(alg(name, levels, 1), 0.9, A, B, 1.1, C, leaf=leaf)
+    for groups in (-1, 1, 4):
+        a = alg(name, levels, 2, groups)
+        assert a.pack_for(m, n, k, leaf) == 2
+        got = run(a, 0.9, A, B, 1.1, C, leaf=leaf)
+        say("bf16x3 wmat", name, levels, leaf, groups)
+        assert np.array_equal(got, want), groups
+    assert alg(name, levels, 1).pack_for(m, n, k, leaf) == 1
