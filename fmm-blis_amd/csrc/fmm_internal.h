@@ -129,12 +129,12 @@ int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
  * (fmmhip_presum.hip): R sums of np parts, every sum a full leaf.  Parts
  * lie at `parts[p].off` from the stage's input, sum r is written at
  * s_off + r*pstride from the stage's output. */
+#define FMMHIP_STAGE_PARTS 16
 typedef struct fmmhip_stage_seg_s {
     long s_off;
     long pstride;
-    int R, np;                  /* np <= 16 */
-    const int* ptr;             /* device CSR over the R sums into ent (absolute indices) */
-    const fmm_entry_t* ent;     /* part = index into parts, coef */
+    int R, np;                  /* np <= FMMHIP_STAGE_PARTS */
+    const double* coef;         /* device, dense [R][FMMHIP_STAGE_PARTS], zero padded */
     const fmm_entry_t* parts;   /* off, rows, cols */
 } fmmhip_stage_seg_t;
 /* One stage: every segment over the leaf layout (Lc, Ls, ldo, swap as in

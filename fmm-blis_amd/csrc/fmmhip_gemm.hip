@@ -253,7 +253,7 @@ bool kron_operand(const struct fmmhip_alg_s* alg, bool is_a)
     const int np = is_a ? alg->mt * alg->kt : alg->kt * alg->nt;
     const int np0 = is_a ? alg->lv_m[0] * alg->lv_k[0] : alg->lv_k[0] * alg->lv_n[0];
     const int np1 = is_a ? alg->lv_m[1] * alg->lv_k[1] : alg->lv_k[1] * alg->lv_n[1];
-    return np >= kKronMinParts && np0 <= FMMHIP_BATCH_PARTS_MAX && np1 <= FMMHIP_BATCH_PARTS_MAX;
+    return np >= kKronMinParts && np0 <= FMMHIP_STAGE_PARTS && np1 <= FMMHIP_STAGE_PARTS;
 }
 
 // Host image of one operand's two-stage tables.  Composed part (row
@@ -264,8 +264,8 @@ bool kron_operand(const struct fmmhip_alg_s* alg, bool is_a)
 // leaf b*R1 + r1 of the stage-1 region.  Stage 2, segment per inner
 // product r1: S[r0*R1 + r1] = sum_b X0[b][r0] Y[b][r1].
 struct KronImage {
-    std::vector<int32_t> ptr0, ptr1;
-    std::vector<fmm_entry_t> ent0, ent1, parts1, parts2;
+    std::vector<double> coef0, coef1;        // dense [R][FMMHIP_STAGE_PARTS] per level
+    std::vector<fmm_entry_t> parts1, parts2;
     std::vector<fmmhip_stage_seg_t> seg1, seg2;
     long x1_elems = 0;
 };
@@ -278,24 +278,14 @@ void kron_image(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool is_a, 
     const int* X0 = is_a ? alg->lv_U[0] : alg->lv_V[0];
     const int* X1 = is_a ? alg->lv_U[1] : alg->lv_V[1];
     const int np0 = rows0 * cols0, np1 = rows1 * cols1;
-    // CSR over products of one level, parts numbered row*cols + col
-    auto csr = [](const int* X, int np, int R, std::vector<int32_t>& ptr, std::vector<fmm_entry_t>& ent) {
-        ptr.assign((size_t)R + 1, 0);
-        for (int r = 0; r < R; r++) {
-            ptr[r] = (int32_t)ent.size();
-            for (int p = 0; p < np; p++)
-                if (X[(size_t)p * R + r]) {
-                    fmm_entry_t e;
-                    memset(&e, 0, sizeof(e));
-                    e.coef = (double)X[(size_t)p * R + r];
-                    e.part = p;
-                    ent.push_back(e);
-                }
-        }
-        ptr[R] = (int32_t)ent.size();
+    // dense coefficients of one level, [r][p] with parts numbered row*cols + col
+    auto dense = [](const int* X, int np, int R, std::vector<double>& out) {
+        out.assign((size_t)R * FMMHIP_STAGE_PARTS, 0.0);
+        for (int r = 0; r < R; r++)
+            for (int p = 0; p < np; p++) out[(size_t)r * FMMHIP_STAGE_PARTS + p] = (double)X[(size_t)p * R + r];
     };
-    csr(X0, np0, R0, k.ptr0, k.ent0);
-    csr(X1, np1, R1, k.ptr1, k.ent1);
+    dense(X0, np0, R0, k.coef0);
+    dense(X1, np1, R1, k.coef1);
     // composed part geometry: the operand's rows and columns split directly
     const long wr = is_a ? g.m : g.k, wc = is_a ? g.k : g.n;
     const int tr = rows0 * rows1, tc = cols0 * cols1;
@@ -356,7 +346,7 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
     const WsLayout wl[2] = {ws_layout_a(g, a_mn), ws_layout_b(g, b_mn)};
     for (int o = 0; o < 2; o++)
         if (use[o]) kron_image(alg, g, o == 0, wl[o], im[o]);
-    // layout of the one allocation: per operand ptr0, ptr1, ent0, ent1, parts1, parts2, seg1, seg2
+    // layout of the one allocation: per operand coef0, coef1, parts1, parts2, seg1, seg2
     std::vector<char>& h = d->kron_host;
     h.clear();
     auto place = [&](const void* src, size_t bytes) {
@@ -369,10 +359,8 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
     for (int o = 0; o < 2; o++) {
         if (!use[o]) continue;
         KronImage& k = im[o];
-        off[o][0] = place(k.ptr0.data(), k.ptr0.size() * sizeof(int32_t));
-        off[o][1] = place(k.ptr1.data(), k.ptr1.size() * sizeof(int32_t));
-        off[o][2] = place(k.ent0.data(), k.ent0.size() * sizeof(fmm_entry_t));
-        off[o][3] = place(k.ent1.data(), k.ent1.size() * sizeof(fmm_entry_t));
+        off[o][0] = place(k.coef0.data(), k.coef0.size() * sizeof(double));
+        off[o][1] = place(k.coef1.data(), k.coef1.size() * sizeof(double));
         off[o][4] = place(k.parts1.data(), k.parts1.size() * sizeof(fmm_entry_t));
         off[o][5] = place(k.parts2.data(), k.parts2.size() * sizeof(fmm_entry_t));
         off[o][6] = place(k.seg1.data(), k.seg1.size() * sizeof(fmmhip_stage_seg_t));
@@ -395,13 +383,11 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
         const int np1 = (int)(k.parts1.size() / k.seg1.size());
         const int np0 = (int)k.seg1.size();
         for (size_t b = 0; b < k.seg1.size(); b++) {
-            s1[b].ptr = (const int*)(dev + off[o][1]);
-            s1[b].ent = (const fmm_entry_t*)(dev + off[o][3]);
+            s1[b].coef = (const double*)(dev + off[o][1]);
             s1[b].parts = (const fmm_entry_t*)(dev + off[o][4]) + b * np1;
         }
         for (size_t r = 0; r < k.seg2.size(); r++) {
-            s2[r].ptr = (const int*)(dev + off[o][0]);
-            s2[r].ent = (const fmm_entry_t*)(dev + off[o][2]);
+            s2[r].coef = (const double*)(dev + off[o][0]);
             s2[r].parts = (const fmm_entry_t*)(dev + off[o][5]) + r * np0;
         }
         const fmmhip_stage_seg_t* d1 = (const fmmhip_stage_seg_t*)(dev + off[o][6]);

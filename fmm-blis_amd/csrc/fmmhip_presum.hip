@@ -199,27 +199,52 @@ __global__ __launch_bounds__(TB) void presum_parts_batch_kernel(const fmmhip_pre
 }
 
 // One stage of the two-stage summation of a two-level algorithm: segment
-// si = blockIdx.x / bps forms its R sums of up to NPMAX parts over the
-// whole leaf (units of TB elements along c of one s, grid-strided over the
-// segment's bps blocks) with the part-staged unit above.
-template <typename T, int NPMAX, int TB>
+// si = blockIdx.x / bps forms its R sums of up to 16 parts over the whole
+// leaf (units of TB elements along c of one s, grid-strided over the
+// segment's bps blocks).  Each thread keeps its element of every part in
+// registers and reads the coefficients as a dense R x 16 table (uniform
+// scalar loads, zero coefficients skipped by a uniform branch): the FMAs
+// run in ascending part order, the order of the sparse lists, with no
+// dependent load per term.
+template <typename T, int TB>
 __global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ X, long ld, T* __restrict__ S,
                                                           int swap, int Lc, int Ls, int ldo,
                                                           const fmmhip_stage_seg_t* __restrict__ segs, int bps)
 {
+    constexpr int NP = FMMHIP_STAGE_PARTS;
     const int si = (int)blockIdx.x / bps;
     const fmmhip_stage_seg_t sg = segs[si];
-    __shared__ double xs[NPMAX][TB];
-    __shared__ long p_off[NPMAX];
-    __shared__ int p_c[NPMAX], p_s[NPMAX];
-    parts_meta<NPMAX, TB>(sg.parts, sg.np, swap, p_off, p_c, p_s);
+    __shared__ long p_off[NP];
+    __shared__ int p_c[NP], p_s[NP];
+    parts_meta<NP, TB>(sg.parts, sg.np, swap, p_off, p_c, p_s);
     __syncthreads();
     const long nbc = (Lc + TB - 1) / TB;
+    T* __restrict__ So = S + sg.s_off;
     for (long u = (long)blockIdx.x - (long)si * bps; u < nbc * Ls; u += bps) {
         const int s = (int)(u / nbc);
         const int c = (int)(u - (long)s * nbc) * TB + threadIdx.x;
-        parts_unit<T, NPMAX, TB>(X, ld, sg.ptr, sg.ent, sg.np, sg.R, Lc, ldo, sg.pstride, S + sg.s_off, s, c, xs,
-                                 p_off, p_c, p_s);
+        double x[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const int p = min(j, sg.np - 1);
+            const int ec = p_c[p], es = p_s[p];
+            const long at = ((ec > 0) & (es > 0)) ? p_off[p] + min(c, ec - 1) + (long)min(s, es - 1) * ld : 0;
+            const double v = (double)X[at];
+            x[j] = ((c < ec) & (s < es) & (j < sg.np)) ? v : 0.0;
+        }
+        if (c < Lc) {
+            T* out = So + c + (long)s * ldo;
+            for (int r = 0; r < sg.R; r++) {
+                const double* cr = sg.coef + (long)r * NP;
+                double v = 0.0;
+#pragma unroll
+                for (int j = 0; j < NP; j++) {
+                    const double cj = cr[j];
+                    if (cj != 0.0) v = fma(cj, x[j], v);
+                }
+                out[(long)r * sg.pstride] = (T)v;
+            }
+        }
     }
 }
 
@@ -260,17 +285,17 @@ extern "C" int fmmhip_presum_stage(int is_double, const void* X, long ld, void* 
 {
     if (nseg <= 0 || Lc <= 0 || Ls <= 0) return FMMHIP_SUCCESS;
     constexpr int TB = 256;
-    // about 8 blocks per CU in all, at most one per unit of the segment
+    // about 65536 blocks in all (grid-strided), at most one per unit of a segment
     const long units = (long)((Lc + TB - 1) / TB) * Ls;
-    const long want = (2048 + nseg - 1) / nseg;
+    const long want = (65536 + nseg - 1) / nseg;
     const int bps = (int)(units < want ? units : want);
     if ((long)bps * nseg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
     if (is_double)
-        hipLaunchKernelGGL((presum_stage_kernel<double, FMMHIP_BATCH_PARTS_MAX, TB>), dim3((unsigned)(bps * nseg)),
+        hipLaunchKernelGGL((presum_stage_kernel<double, TB>), dim3((unsigned)(bps * nseg)),
                            dim3(TB), 0, (hipStream_t)stream, (const double*)X, ld, (double*)S, swap, Lc, Ls, ldo,
                            segs, bps);
     else
-        hipLaunchKernelGGL((presum_stage_kernel<float, FMMHIP_BATCH_PARTS_MAX, TB>), dim3((unsigned)(bps * nseg)),
+        hipLaunchKernelGGL((presum_stage_kernel<float, TB>), dim3((unsigned)(bps * nseg)),
                            dim3(TB), 0, (hipStream_t)stream, (const float*)X, ld, (float*)S, swap, Lc, Ls, ldo, segs,
                            bps);
     return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
