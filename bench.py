@@ -557,6 +557,9 @@ def main():
     mfma_fl = fl * {"bf16x3": 8, "bf16x3_6": 6}.get(args.leaf, 1)
     achieved = mfma_fl / (kern_ms * 1e-3) / 1e12
     pack = {0: "fused", 1: "pre-summed", 2: "pre-summed, products materialised"}[pack_mode]
+    # the counter summary of this workload and mode (profiles/traffic_<tag>.txt)
+    tag = f"{args.alg}L{args.levels}{ {0: 'fused', 1: '', 2: 'mat'}[pack_mode] if args.alg != 'classical' else ''}" \
+          f"_{args.leaf}_{n}"
     if args.leaf.startswith("bf16x3"):
         pack = "bf16 planes (split stage)"
     out = {

@@ -586,11 +586,9 @@ int mat_groups(int R, long tiles, int slots)
     return best;
 }
 
-// Whether the pre-summed mode is expected to beat the products-materialised
-// one by 3% (the automatic choice otherwise materialises: measured on
-// 222, 333, 444 at 1 and 2 levels from 4096^3 to 16384^3 it is never more
-// than 2% slower and up to 29% faster, its grid filling the device with
-// groups that need no copies of C).  Time model (both share the summation stage): the leaf
+// Whether the pre-summed mode (W epilogue fused, no product in HBM) is
+// expected to be at least as fast as the products-materialised one.  Time
+// model (both share the summation stage): the leaf
 // flops at a sustained MFMA rate divided by the wave fill of each mode's
 // grid (pre-summed: the split its own model picks, copies of C charged;
 // materialised: mat_groups, no copies), plus, pre-summed, the fused W
@@ -618,7 +616,7 @@ bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, si
                          (std::ceil((double)tiles * g2 / slots) * std::ceil((double)R / g2));
     const double t2 = leaf_flops / (rate * std::min(fill2, 1.0)) +
                       ((double)R * leaf_c + (double)alg->nnz_w * leaf_c + 2.0 * (double)esz * c_span) / 4.9e12;
-    return t1 < 0.97 * t2;
+    return t1 <= t2;
 }
 
 // The pack mode of one call: 0 fused, 1 pre-summed, 2 pre-summed with the
@@ -626,7 +624,7 @@ bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, si
 // several products and the one-problem path, whose lists it builds), or
 // automatic: pre-summed for multi-part algorithms when its workspace fits
 // in half of the free device memory, materialised as well when the slots
-// fit too and the pre-summed mode is not expected to win (presum_wins).
+// fit too and the time model expects it to be faster (presum_wins).
 int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int split, size_t esz, size_t ws_bytes,
                  double c_span, bool one_problem)
 {

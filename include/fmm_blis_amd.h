@@ -233,7 +233,7 @@ int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
  * batch uses 1 instead); -1 = chosen per call (default: for multi-part
  * algorithms pre-summed when its workspace fits in half of the free
  * device memory, materialised too when that fits as well and a time model
- * does not favour 1).  No counterpart in the reference, whose packing
+ * expects it to be faster).  No counterpart in the reference, whose packing
  * (plugin/ref_kernels/bli_packm_fmm_ref.c) and scatter
  * (plugin/ref_kernels/bli_gemm_fmm_ref.c) are always fused. */
 int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
