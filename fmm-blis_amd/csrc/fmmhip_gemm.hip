@@ -628,7 +628,8 @@ bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, si
 int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int split, size_t esz, size_t ws_bytes,
                  double c_span, bool one_problem)
 {
-    const bool can_mat = alg->R > 1 && one_problem;
+    // the slots' leading dimension is a kernel stride (32-bit tile offsets)
+    const bool can_mat = alg->R > 1 && one_problem && g.lm <= (1L << 23) && g.ln <= (1L << 23);
     if (pack >= 0) return pack == 2 ? (can_mat ? 2 : 1) : pack;
     if (alg->max_nnz_u <= 1 && alg->max_nnz_v <= 1) return 0;
     size_t fr = 0, tot = 0;
