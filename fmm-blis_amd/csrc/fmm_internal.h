@@ -130,11 +130,17 @@ int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
  * lie at `parts[p].off` from the stage's input, sum r is written at
  * s_off + r*pstride from the stage's output. */
 #define FMMHIP_STAGE_PARTS 16
+typedef struct fmmhip_stage_term_s {
+    double coef;
+    int32_t part;               /* < np */
+    int32_t pad;
+} fmmhip_stage_term_t;
 typedef struct fmmhip_stage_seg_s {
     long s_off;
     long pstride;
     int R, np;                  /* np <= FMMHIP_STAGE_PARTS */
-    const double* coef;         /* device, dense [R][FMMHIP_STAGE_PARTS], zero padded */
+    const int* tptr;            /* device CSR over the R sums into terms (ascending part per sum) */
+    const fmmhip_stage_term_t* terms;
     const fmm_entry_t* parts;   /* off, rows, cols */
 } fmmhip_stage_seg_t;
 /* One stage: every segment over the leaf layout (Lc, Ls, ldo, swap as in

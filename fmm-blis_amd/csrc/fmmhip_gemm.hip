@@ -264,7 +264,8 @@ bool kron_operand(const struct fmmhip_alg_s* alg, bool is_a)
 // leaf b*R1 + r1 of the stage-1 region.  Stage 2, segment per inner
 // product r1: S[r0*R1 + r1] = sum_b X0[b][r0] Y[b][r1].
 struct KronImage {
-    std::vector<double> coef0, coef1;        // dense [R][FMMHIP_STAGE_PARTS] per level
+    std::vector<int32_t> tptr0, tptr1;       // CSR over products per level
+    std::vector<fmmhip_stage_term_t> term0, term1;
     std::vector<fmm_entry_t> parts1, parts2;
     std::vector<fmmhip_stage_seg_t> seg1, seg2;
     long x1_elems = 0;
@@ -278,14 +279,24 @@ void kron_image(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool is_a, 
     const int* X0 = is_a ? alg->lv_U[0] : alg->lv_V[0];
     const int* X1 = is_a ? alg->lv_U[1] : alg->lv_V[1];
     const int np0 = rows0 * cols0, np1 = rows1 * cols1;
-    // dense coefficients of one level, [r][p] with parts numbered row*cols + col
-    auto dense = [](const int* X, int np, int R, std::vector<double>& out) {
-        out.assign((size_t)R * FMMHIP_STAGE_PARTS, 0.0);
-        for (int r = 0; r < R; r++)
-            for (int p = 0; p < np; p++) out[(size_t)r * FMMHIP_STAGE_PARTS + p] = (double)X[(size_t)p * R + r];
+    // nonzero terms of one level per product, ascending part (numbered row*cols + col)
+    auto sparse = [](const int* X, int np, int R, std::vector<int32_t>& ptr, std::vector<fmmhip_stage_term_t>& tm) {
+        ptr.assign((size_t)R + 1, 0);
+        for (int r = 0; r < R; r++) {
+            ptr[r] = (int32_t)tm.size();
+            for (int p = 0; p < np; p++)
+                if (X[(size_t)p * R + r]) {
+                    fmmhip_stage_term_t t;
+                    memset(&t, 0, sizeof(t));
+                    t.coef = (double)X[(size_t)p * R + r];
+                    t.part = p;
+                    tm.push_back(t);
+                }
+        }
+        ptr[R] = (int32_t)tm.size();
     };
-    dense(X0, np0, R0, k.coef0);
-    dense(X1, np1, R1, k.coef1);
+    sparse(X0, np0, R0, k.tptr0, k.term0);
+    sparse(X1, np1, R1, k.tptr1, k.term1);
     // composed part geometry: the operand's rows and columns split directly
     const long wr = is_a ? g.m : g.k, wc = is_a ? g.k : g.n;
     const int tr = rows0 * rows1, tc = cols0 * cols1;
@@ -346,7 +357,7 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
     const WsLayout wl[2] = {ws_layout_a(g, a_mn), ws_layout_b(g, b_mn)};
     for (int o = 0; o < 2; o++)
         if (use[o]) kron_image(alg, g, o == 0, wl[o], im[o]);
-    // layout of the one allocation: per operand coef0, coef1, parts1, parts2, seg1, seg2
+    // layout of the one allocation: per operand tptr0, tptr1, term0, term1, parts1, parts2, seg1, seg2
     std::vector<char>& h = d->kron_host;
     h.clear();
     auto place = [&](const void* src, size_t bytes) {
@@ -359,8 +370,10 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
     for (int o = 0; o < 2; o++) {
         if (!use[o]) continue;
         KronImage& k = im[o];
-        off[o][0] = place(k.coef0.data(), k.coef0.size() * sizeof(double));
-        off[o][1] = place(k.coef1.data(), k.coef1.size() * sizeof(double));
+        off[o][0] = place(k.tptr0.data(), k.tptr0.size() * sizeof(int32_t));
+        off[o][1] = place(k.tptr1.data(), k.tptr1.size() * sizeof(int32_t));
+        off[o][2] = place(k.term0.data(), k.term0.size() * sizeof(fmmhip_stage_term_t));
+        off[o][3] = place(k.term1.data(), k.term1.size() * sizeof(fmmhip_stage_term_t));
         off[o][4] = place(k.parts1.data(), k.parts1.size() * sizeof(fmm_entry_t));
         off[o][5] = place(k.parts2.data(), k.parts2.size() * sizeof(fmm_entry_t));
         off[o][6] = place(k.seg1.data(), k.seg1.size() * sizeof(fmmhip_stage_seg_t));
@@ -383,11 +396,13 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
         const int np1 = (int)(k.parts1.size() / k.seg1.size());
         const int np0 = (int)k.seg1.size();
         for (size_t b = 0; b < k.seg1.size(); b++) {
-            s1[b].coef = (const double*)(dev + off[o][1]);
+            s1[b].tptr = (const int*)(dev + off[o][1]);
+            s1[b].terms = (const fmmhip_stage_term_t*)(dev + off[o][3]);
             s1[b].parts = (const fmm_entry_t*)(dev + off[o][4]) + b * np1;
         }
         for (size_t r = 0; r < k.seg2.size(); r++) {
-            s2[r].coef = (const double*)(dev + off[o][0]);
+            s2[r].tptr = (const int*)(dev + off[o][0]);
+            s2[r].terms = (const fmmhip_stage_term_t*)(dev + off[o][2]);
             s2[r].parts = (const fmm_entry_t*)(dev + off[o][5]) + r * np0;
         }
         const fmmhip_stage_seg_t* d1 = (const fmmhip_stage_seg_t*)(dev + off[o][6]);

@@ -200,12 +200,12 @@ __global__ __launch_bounds__(TB) void presum_parts_batch_kernel(const fmmhip_pre
 
 // One stage of the two-stage summation of a two-level algorithm: segment
 // si = blockIdx.x / bps forms its R sums of up to 16 parts over the whole
-// leaf (units of TB elements along c of one s, grid-strided over the
-// segment's bps blocks).  Each thread keeps its element of every part in
-// registers and reads the coefficients as a dense R x 16 table (uniform
-// scalar loads, zero coefficients skipped by a uniform branch): the FMAs
-// run in ascending part order, the order of the sparse lists, with no
-// dependent load per term.
+// leaf (units of 2*TB elements along c of one s, grid-strided over the
+// segment's bps blocks).  Each thread holds two adjacent elements of every
+// part in registers; the sums walk the segment's sparse term list (part,
+// coefficient; ascending part per sum, uniform scalar loads), the part
+// register chosen by a uniform switch, so only nonzero terms cost VALU
+// work and no load depends on a term.
 template <typename T, int TB>
 __global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ X, long ld, T* __restrict__ S,
                                                           int swap, int Lc, int Ls, int ldo,
@@ -218,31 +218,47 @@ __global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ 
     __shared__ int p_c[NP], p_s[NP];
     parts_meta<NP, TB>(sg.parts, sg.np, swap, p_off, p_c, p_s);
     __syncthreads();
-    const long nbc = (Lc + TB - 1) / TB;
+    const long nbc = (Lc + 2 * TB - 1) / (2 * TB);
     T* __restrict__ So = S + sg.s_off;
     for (long u = (long)blockIdx.x - (long)si * bps; u < nbc * Ls; u += bps) {
         const int s = (int)(u / nbc);
-        const int c = (int)(u - (long)s * nbc) * TB + threadIdx.x;
-        double x[NP];
+        const int c0 = (int)(u - (long)s * nbc) * 2 * TB + 2 * (int)threadIdx.x;
+        double x0[NP], x1[NP];
 #pragma unroll
         for (int j = 0; j < NP; j++) {
             const int p = min(j, sg.np - 1);
             const int ec = p_c[p], es = p_s[p];
-            const long at = ((ec > 0) & (es > 0)) ? p_off[p] + min(c, ec - 1) + (long)min(s, es - 1) * ld : 0;
-            const double v = (double)X[at];
-            x[j] = ((c < ec) & (s < es) & (j < sg.np)) ? v : 0.0;
+            const bool live = (ec > 0) & (es > 0);
+            const long row = p_off[p] + (long)min(s, es - 1) * ld;
+            const double v0 = (double)X[live ? row + min(c0, ec - 1) : 0];
+            const double v1 = (double)X[live ? row + min(c0 + 1, ec - 1) : 0];
+            const bool in = (s < es) & (j < sg.np);
+            x0[j] = (in & (c0 < ec)) ? v0 : 0.0;
+            x1[j] = (in & (c0 + 1 < ec)) ? v1 : 0.0;
         }
-        if (c < Lc) {
-            T* out = So + c + (long)s * ldo;
+        if (c0 < Lc) {
+            T* out = So + c0 + (long)s * ldo;
+            const bool two = c0 + 1 < Lc;
             for (int r = 0; r < sg.R; r++) {
-                const double* cr = sg.coef + (long)r * NP;
-                double v = 0.0;
-#pragma unroll
-                for (int j = 0; j < NP; j++) {
-                    const double cj = cr[j];
-                    if (cj != 0.0) v = fma(cj, x[j], v);
+                double v0 = 0.0, v1 = 0.0;
+                for (int t = sg.tptr[r]; t < sg.tptr[r + 1]; t++) {
+                    const fmmhip_stage_term_t tm = sg.terms[t];
+                    switch (tm.part) {
+#define STAGE_TERM(K)                                                                                         \
+    case K:                                                                                                   \
+        v0 = fma(tm.coef, x0[K], v0);                                                                         \
+        v1 = fma(tm.coef, x1[K], v1);                                                                         \
+        break;
+                        STAGE_TERM(0) STAGE_TERM(1) STAGE_TERM(2) STAGE_TERM(3)
+                        STAGE_TERM(4) STAGE_TERM(5) STAGE_TERM(6) STAGE_TERM(7)
+                        STAGE_TERM(8) STAGE_TERM(9) STAGE_TERM(10) STAGE_TERM(11)
+                        STAGE_TERM(12) STAGE_TERM(13) STAGE_TERM(14) STAGE_TERM(15)
+#undef STAGE_TERM
+                    default: break;
+                    }
                 }
-                out[(long)r * sg.pstride] = (T)v;
+                out[(long)r * sg.pstride] = (T)v0;
+                if (two) out[(long)r * sg.pstride + 1] = (T)v1;
             }
         }
     }
@@ -286,7 +302,7 @@ extern "C" int fmmhip_presum_stage(int is_double, const void* X, long ld, void* 
     if (nseg <= 0 || Lc <= 0 || Ls <= 0) return FMMHIP_SUCCESS;
     constexpr int TB = 256;
     // about 65536 blocks in all (grid-strided), at most one per unit of a segment
-    const long units = (long)((Lc + TB - 1) / TB) * Ls;
+    const long units = (long)((Lc + 2 * TB - 1) / (2 * TB)) * Ls;
     const long want = (65536 + nseg - 1) / nseg;
     const int bps = (int)(units < want ? units : want);
     if ((long)bps * nseg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
