@@ -570,7 +570,7 @@ def main():
     tag = f"{args.alg}L{args.levels}{ {0: 'fused', 1: '', 2: 'mat'}[pack_mode] if args.alg != 'classical' else ''}" \
           f"_{args.leaf}_{n}"
     if args.leaf.startswith("bf16x3"):
-        pack = "bf16 planes (split stage)"
+        pack = "bf16 planes (split stage)" + (", products materialised" if pack_mode == 2 else "")
     out = {
         "metric": "effective FP64 GFLOPS (2mnk/t) FMM vs classical at 16384^3; p99 GEMM latency",
         "value": round(eff, 1),
