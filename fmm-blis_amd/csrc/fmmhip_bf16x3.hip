@@ -468,6 +468,9 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     g.rsA = rsA; g.csA = csA; g.rsB = rsB; g.csB = csB; g.rsC = rsC; g.csC = csC;
     const int* dptr = nullptr;
     const fmm_entry_t* dent = nullptr;
+    // held from the descriptor lookups to the last launch: another call
+    // cannot evict the cached lists or regrow the workspace in between
+    std::unique_lock<std::mutex> lk = enqueue_lock();
     int rc = fused_desc(alg, g, st, &dptr, &dent);
     if (rc) return rc;
 
@@ -504,7 +507,6 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const long tmp_elems = (!wmat && groups > 1) ? (((m - 1) * rsC + (n - 1) * csC + 1 + 31) & ~31L) : 0;
     const size_t oT = oEB + al(expB);        // split temporaries, or the product slots
     const size_t bytes = oT + (wmat ? (size_t)R * (size_t)mv.slot * 8 : (size_t)groups * tmp_elems * 8);
-    std::unique_lock<std::mutex> lk = enqueue_lock();   // held until the last launch below
     char* ws = (char*)workspace_for(st, bytes);
     if (!ws) return FMMHIP_ERR_NOMEM;
 
