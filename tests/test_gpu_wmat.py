@@ -164,3 +164,24 @@ def test_bf16x3_materialised_equals_in_kernel_epilogue(name, levels, leaf):
         say("bf16x3 wmat", name, levels, leaf, groups)
         assert np.array_equal(got, want), groups
     assert alg(name, levels, 1).pack_for(m, n, k, leaf) == 1
+
+
+@pytest.mark.parametrize("leaf", [F.LEAF_NATIVE, F.LEAF_BF16X3])
+def test_wmat_leaves_gaps_of_c_untouched(leaf):
+    """C with a leading dimension above m: the W stage writes the parts'
+    elements only, so the gap rows keep their contents; the result is the
+    oracle's FMM bit for bit on small integers (FP64 and BF16x3 leaves)."""
+    m, n, k, ldc = 157, 149, 133, 168
+    og = O.parse(alg_path("222"))
+    og = O.compose(og, og)
+    A, B, C = O.integers(m, k, 47, 0, 3), O.integers(k, n, 47, 1, 3), O.integers(m, n, 47, 2, 3)
+    want = O.fmm_dgemm(og, 1.0, A, B, -1.0, C.copy(order="F"))
+    full = np.full((n, ldc), 7.25)
+    full[:, :m] = C.T
+    dC = F.DeviceMatrix(m, n, ld=ldc)
+    dC.buf.upload(full)
+    F.dgemm_fmm(alg("222", 2, 2, -1), m, n, k, 1.0, dev(A), dev(B), -1.0, dC, leaf=leaf)
+    F.sync()
+    got = dC.buf.download(np.float64, ldc * n).reshape(n, ldc)
+    assert np.all(got[:, m:] == 7.25)
+    assert np.array_equal(got[:, :m].T, want)
