@@ -45,7 +45,7 @@ struct fmmhip_alg_s {
     /* two-level algorithms: the two levels' own tables (level 0 outer),
      * for the two-stage operand summation; NULL otherwise */
     int  lv_m[2], lv_k[2], lv_n[2], lv_R[2];
-    int *lv_U[2], *lv_V[2];     /* dense [part][r] */
+    int *lv_U[2], *lv_V[2], *lv_W[2];   /* dense [part][r] */
     void *dev_cache;            /* owned by the HIP layer (fmmhip_alg_release_cache) */
 };
 

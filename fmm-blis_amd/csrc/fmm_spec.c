@@ -290,7 +290,7 @@ void fmmhip_alg_destroy(fmmhip_alg_t g)
     free(g->u_ptr); free(g->u_part); free(g->u_coef);
     free(g->v_ptr); free(g->v_part); free(g->v_coef);
     free(g->w_ptr); free(g->w_part); free(g->w_coef); free(g->w_first);
-    for (int l = 0; l < 2; l++) { free(g->lv_U[l]); free(g->lv_V[l]); }
+    for (int l = 0; l < 2; l++) { free(g->lv_U[l]); free(g->lv_V[l]); free(g->lv_W[l]); }
     free(g);
 }
 
@@ -347,13 +347,16 @@ int fmmhip_alg_create(fmmhip_alg_t* out, const fmm_t* algs, int nlevels)
     if (nlevels == 2) {
         for (int l = 0; l < 2; l++) {
             const fmm_t* b = &algs[l];
-            const size_t nu = (size_t)b->m_tilde * b->k_tilde * b->R, nv = (size_t)b->k_tilde * b->n_tilde * b->R;
+            const size_t nu = (size_t)b->m_tilde * b->k_tilde * b->R, nv = (size_t)b->k_tilde * b->n_tilde * b->R,
+                         nw = (size_t)b->m_tilde * b->n_tilde * b->R;
             g->lv_m[l] = b->m_tilde; g->lv_k[l] = b->k_tilde; g->lv_n[l] = b->n_tilde; g->lv_R[l] = b->R;
             g->lv_U[l] = (int*)malloc(sizeof(int) * nu);
             g->lv_V[l] = (int*)malloc(sizeof(int) * nv);
-            if (!g->lv_U[l] || !g->lv_V[l]) { fmmhip_alg_destroy(g); return FMMHIP_ERR_NOMEM; }
+            g->lv_W[l] = (int*)malloc(sizeof(int) * nw);
+            if (!g->lv_U[l] || !g->lv_V[l] || !g->lv_W[l]) { fmmhip_alg_destroy(g); return FMMHIP_ERR_NOMEM; }
             memcpy(g->lv_U[l], b->U, sizeof(int) * nu);
             memcpy(g->lv_V[l], b->V, sizeof(int) * nv);
+            memcpy(g->lv_W[l], b->W, sizeof(int) * nw);
         }
     }
     g->classical = (cm == 1 && ck == 1 && cn == 1 && cR == 1);

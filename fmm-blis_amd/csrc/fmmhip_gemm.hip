@@ -43,6 +43,10 @@ struct DevDesc {
     std::vector<char> kron_host; // their host image (source of the upload)
     KronView kron{};
     bool has_kron = false;
+    void* wk_dev = nullptr;      // two-stage W stage tables (one allocation)
+    std::vector<char> wk_host;
+    WKronView wk{};
+    bool has_wk = false;
     unsigned long last_use = 0;
 };
 
@@ -74,6 +78,7 @@ void free_desc(DevDesc* d)
     free_list(d->presum);
     free_list(d->mat);
     if (d->kron_dev) (void)hipFree(d->kron_dev);
+    if (d->wk_dev) (void)hipFree(d->wk_dev);
     delete d;
 }
 
@@ -419,6 +424,110 @@ int build_kron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, bool a_mn, b
     return FMMHIP_SUCCESS;
 }
 
+// Two-stage W stage of the products-materialised mode (two-level
+// algorithms): stage 1 forms, for every outer product r0 and inner C part
+// c1, Y[r0][c1] = sum_r1 W1[c1][r1] M[r0*R1 + r1] (a full leaf, r1
+// ascending); stage 2 forms composed C part (c0, c1) := beta C + alpha
+// sum_r0 W0[c0][r0] Y[r0][c1] (r0 ascending).  Taken when it moves fewer
+// leaves than the one-pass stage: R0*nnz(W1) slot reads + R0*np1 Y writes
+// + nnz(W0)*np1 Y reads against nnz(W) slot reads (C's read and write are
+// common to both).
+bool wkron_pays(const struct fmmhip_alg_s* alg)
+{
+    if (alg->levels != 2 || !alg->lv_W[0] || !alg->lv_W[1]) return false;
+    long nnz[2] = {0, 0};
+    for (int l = 0; l < 2; l++) {
+        const long cells = (long)alg->lv_m[l] * alg->lv_n[l] * alg->lv_R[l];
+        for (long i = 0; i < cells; i++) nnz[l] += alg->lv_W[l][i] != 0;
+    }
+    const long np1 = (long)alg->lv_m[1] * alg->lv_n[1];
+    const long two = (long)alg->lv_R[0] * nnz[1] + (long)alg->lv_R[0] * np1 + nnz[0] * np1;
+    return two < alg->nnz_w;
+}
+
+// Host image and upload of the two-stage W tables, from the materialised
+// list's C part table (off, rows, cols).
+int build_wkron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, DevDesc* d, hipStream_t stream)
+{
+    const int R = alg->R, R0 = alg->lv_R[0], R1 = alg->lv_R[1];
+    const int rows0 = alg->lv_m[0], cols0 = alg->lv_n[0], rows1 = alg->lv_m[1], cols1 = alg->lv_n[1];
+    const int np0 = rows0 * cols0, np1 = rows1 * cols1, npc = alg->mt * alg->nt, nt = alg->nt;
+    const int* W0 = alg->lv_W[0];
+    const int* W1 = alg->lv_W[1];
+    const long slot = slot_layout(g).slot;
+    const fmm_entry_t* cparts = d->mat.ent_h + 3 * R;
+    std::vector<fmm_entry_t> p1((size_t)R0 * np1), t1, p2(cparts, cparts + npc), t2;
+    for (int r0 = 0; r0 < R0; r0++)
+        for (int c1 = 0; c1 < np1; c1++) {
+            fmm_entry_t& pe = p1[(size_t)r0 * np1 + c1];
+            memset(&pe, 0, sizeof(pe));
+            pe.off = ((long)r0 * np1 + c1) * slot;
+            pe.rows = (int32_t)g.lm;
+            pe.cols = (int32_t)g.ln;
+            pe.first = (int32_t)t1.size();
+            for (int r1 = 0; r1 < R1; r1++) {
+                const int w = W1[(size_t)c1 * R1 + r1];
+                if (!w) continue;
+                fmm_entry_t t;
+                memset(&t, 0, sizeof(t));
+                t.off = ((long)r0 * R1 + r1) * slot;
+                t.coef = (double)w;
+                t.part = r0 * R1 + r1;
+                t1.push_back(t);
+            }
+            pe.part = (int32_t)t1.size() - pe.first;
+        }
+    for (int a0 = 0; a0 < rows0; a0++)
+        for (int b0 = 0; b0 < cols0; b0++)
+            for (int a1 = 0; a1 < rows1; a1++)
+                for (int b1 = 0; b1 < cols1; b1++) {
+                    const int c0 = a0 * cols0 + b0, c1 = a1 * cols1 + b1;
+                    fmm_entry_t& pe = p2[(size_t)(a0 * rows1 + a1) * nt + (b0 * cols1 + b1)];
+                    pe.first = (int32_t)t2.size();
+                    for (int r0 = 0; r0 < R0; r0++) {
+                        const int w = W0[(size_t)c0 * R0 + r0];
+                        if (!w) continue;
+                        fmm_entry_t t;
+                        memset(&t, 0, sizeof(t));
+                        t.off = ((long)r0 * np1 + c1) * slot;
+                        t.coef = (double)w;
+                        t.part = r0;
+                        t2.push_back(t);
+                    }
+                    pe.part = (int32_t)t2.size() - pe.first;
+                }
+    (void)np0;
+    std::vector<char>& h = d->wk_host;
+    h.clear();
+    auto place = [&](const void* src, size_t bytes) {
+        const size_t off = (h.size() + 255) & ~(size_t)255;
+        h.resize(off + bytes);
+        if (bytes) memcpy(h.data() + off, src, bytes);
+        return off;
+    };
+    const size_t o1 = place(p1.data(), p1.size() * sizeof(fmm_entry_t));
+    const size_t o2 = place(t1.data(), t1.size() * sizeof(fmm_entry_t));
+    const size_t o3 = place(p2.data(), p2.size() * sizeof(fmm_entry_t));
+    const size_t o4 = place(t2.data(), t2.size() * sizeof(fmm_entry_t));
+    if (hipMalloc(&d->wk_dev, h.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        d->wk_dev = nullptr;
+        return FMMHIP_ERR_NOMEM;
+    }
+    char* dev = (char*)d->wk_dev;
+    WKronView& v = d->wk;
+    v.parts1 = (const fmm_entry_t*)(dev + o1);
+    v.terms1 = (const fmm_entry_t*)(dev + o2);
+    v.parts2 = (const fmm_entry_t*)(dev + o3);
+    v.terms2 = (const fmm_entry_t*)(dev + o4);
+    v.np1 = R0 * np1;
+    v.np2 = npc;
+    v.y_leaves = (long)R0 * np1;
+    if (hipMemcpyAsync(d->wk_dev, h.data(), h.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return FMMHIP_ERR_HIP;
+    return FMMHIP_SUCCESS;
+}
+
 int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool want_mat, bool a_mn, bool b_mn,
              hipStream_t stream, DevDesc** out)
 {
@@ -471,6 +580,16 @@ int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool want_
         if (!rc) rc = upload_list(d->mat, alg->R, stream);
         if (rc) { free_list(d->mat); return rc; }
         d->has_mat = true;
+    }
+    if (want_mat && !d->has_wk && wkron_pays(alg)) {
+        int rc = build_wkron(alg, g, d, stream);
+        if (rc) {
+            if (d->wk_dev) (void)hipFree(d->wk_dev);
+            d->wk_dev = nullptr;
+            memset(&d->wk, 0, sizeof(d->wk));
+            return rc;
+        }
+        d->has_wk = true;
     }
     d->last_use = cache->clock;
     *out = d;
@@ -723,6 +842,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
             if (wmat) {
                 dv.mat_ptr = d->mat.dptr();
                 dv.mat_ent = d->mat.dent();
+                dv.wk = d->wk;
             }
             if (presum) dv.kron = d->kron;
         }
@@ -833,6 +953,26 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         w.np = alg->mt * alg->nt;
         w.parts = dv.mat_ent + 3 * alg->R;
         w.terms = w.parts + w.np;
+        if (dv.wk.parts1) {
+            // two-stage W: stage 1 into the Y leaves (after the slots), stage 2 into C
+            P->w2 = true;
+            P->y_off = (P->ws_bytes + 255) & ~(size_t)255;
+            P->ws_bytes = P->y_off + esz * (size_t)sl.slot * (size_t)dv.wk.y_leaves;
+            fmmhip_wsum_t& w2 = P->wsum2;
+            w2 = w;                              // stage 2: from the Y leaves into C
+            w2.M = (void*)(uintptr_t)P->y_off;
+            w2.np = dv.wk.np2;
+            w2.parts = dv.wk.parts2;
+            w2.terms = dv.wk.terms2;
+            w.C = (void*)(uintptr_t)P->y_off;    // stage 1: from the slots into the Y leaves
+            w.rsC = ka.rsC;
+            w.csC = ka.csC;
+            w.alpha = 1.0;
+            w.beta = 0.0;
+            w.np = dv.wk.np1;
+            w.parts = dv.wk.parts1;
+            w.terms = dv.wk.terms1;
+        }
         P->nwg = tiles * ka.groups;
         if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
         if (tiles <= 0) P->nothing = true;
@@ -866,6 +1006,10 @@ void bind_workspace(Prepared* P, void* ws)
         P->ka.C = base + P->m_off;
         P->ka.tmp = P->ka.C;
         P->wsum.M = P->ka.C;
+        if (P->w2) {
+            P->wsum.C = base + P->y_off;
+            P->wsum2.M = base + P->y_off;
+        }
     } else if (P->ka.groups > 1) {
         P->ka.tmp = base + P->tmp_off;
     }
@@ -935,6 +1079,7 @@ int launch_prepared(const Prepared* P)
         rc = fmmhip_split_reduce(dbl, P->C, P->m, P->n, P->rsC, P->csC, P->beta, P->ka.tmp, P->ka.tmp_elems,
                                  P->ka.groups, st);
     if (!rc && P->wmat) rc = fmmhip_wsum(dbl, &P->wsum, st);
+    if (!rc && P->w2) rc = fmmhip_wsum(dbl, &P->wsum2, st);
     if (timed) {
         if (split || P->wmat) tm_record(&tc.e3, st);
         std::lock_guard<std::mutex> lk(g_tm_mu);

@@ -21,6 +21,17 @@ struct KronView {
     long x1_elems;               // stage-1 leaves (the larger operand's)
 };
 
+// Two-stage W stage of the products-materialised mode (device tables):
+// stage 1 forms, per outer product and inner C part, the inner level's
+// W-weighted sum of the slots (the Y leaves); stage 2 forms each C part
+// from the Y leaves with the outer level's W.
+struct WKronView {
+    const fmm_entry_t *parts1, *terms1;   // Y leaves (R0 x inner parts) and their terms
+    const fmm_entry_t *parts2, *terms2;   // C parts and their terms over the Y leaves
+    int np1, np2;
+    long y_leaves;
+};
+
 struct Prepared {
     Kind kind;
     hipStream_t stream;
@@ -43,6 +54,9 @@ struct Prepared {
     bool wmat;                   // products materialised: kernel into slots, then the W stage
     size_t m_off;                // slots: workspace offset
     fmmhip_wsum_t wsum;          // the W stage's arguments (slots bound with the workspace)
+    bool w2;                     // two-stage W: wsum into the Y leaves, then wsum2 into C
+    size_t y_off;                // the Y leaves: workspace offset
+    fmmhip_wsum_t wsum2;
 };
 
 // Device coefficient lists of one call: the fused list (every nonzero part
@@ -57,6 +71,7 @@ struct DescView {
     const int* mat_ptr;          // products-materialised list (one-problem path only)
     const fmm_entry_t* mat_ent;
     KronView kron;               // two-stage summation tables (one-problem path only)
+    WKronView wk;                // two-stage W stage tables (materialised, one-problem path only)
 };
 
 // Where prepare_call takes the lists from; without one, from the

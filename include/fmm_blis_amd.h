@@ -229,8 +229,10 @@ int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
  * bandwidth pass forms each C part from the products it needs: one pass
  * per part instead of one read-modify-write of C per (product, part), and
  * any number of workgroups per tile, up to R, without copies of C; the
- * same sums in the same order as 1 without split, so the same C; the
- * batch uses 1 instead); -1 = chosen per call (default: for multi-part
+ * same sums in the same order as 1 without split, so the same C, except
+ * for two-level tables whose W stage runs in two stages, inner level then
+ * outer (333, 444: fewer bytes, sums re-associated); the batch uses 1
+ * instead); -1 = chosen per call (default: for multi-part
  * algorithms pre-summed when its workspace fits in half of the free
  * device memory, materialised too when that fits as well and a time model
  * expects it to be faster).  No counterpart in the reference, whose packing

@@ -9,7 +9,9 @@ Bars (stated per test):
   * uniform[-1,1] inputs: equal to the pre-summed mode (pack 1, no split)
     bit for bit, for every group count of the product split, FP64, FP32
     operands and the FP32 leaf -- the two modes run the same sums in the
-    same order;
+    same order -- except two-level 333 and 444, whose W stage runs in two
+    stages (inner level, then outer): there within the per-level error
+    bound, and exact on integers;
   * beta == 0 never reads C (NaN in C does not propagate).
 """
 import sys
@@ -66,6 +68,9 @@ def size_for(name, levels):
     return {("444", 2): 150, ("333", 2): 200}.get((name, levels), 333)
 
 
+TWO_STAGE_W = {("333", 2), ("444", 2)}
+
+
 @pytest.mark.parametrize("name", ALG_NAMES)
 @pytest.mark.parametrize("levels", [1, 2])
 def test_wmat_equals_presum_bitwise(name, levels):
@@ -75,8 +80,13 @@ def test_wmat_equals_presum_bitwise(name, levels):
     want = run(alg(name, levels, 1), 0.7, A, B, -1.3, C)
     got = run(alg(name, levels, 2, -1), 0.7, A, B, -1.3, C)
     say("wmat vs presum", name, levels, m, n, k)
-    assert np.array_equal(got, want)
     ref = O.dgemm(0.7, A, B, -1.3, C.copy(order="F"))
+    if (name, levels) in TWO_STAGE_W:
+        # the W sums re-associated: both within the bound, and close
+        assert O.normalized_error(want, ref, A, B) < 5e-15
+        assert O.normalized_error(got, want, A, B) < 5e-15
+    else:
+        assert np.array_equal(got, want)
     assert O.normalized_error(got, ref, A, B) < 5e-15
 
 
@@ -185,3 +195,27 @@ def test_wmat_leaves_gaps_of_c_untouched(leaf):
     got = dC.buf.download(np.float64, ldc * n).reshape(n, ldc)
     assert np.all(got[:, m:] == 7.25)
     assert np.array_equal(got[:, :m].T, want)
+
+
+@pytest.mark.parametrize("name", ["333", "444"])
+def test_two_stage_w_exact_integers_row_major_c_and_nan(name):
+    """Two-stage W stage (333 and 444 at two levels): exact against the
+    oracle's FMM on small integers with C stored by columns and by rows,
+    alpha/beta pairs, and beta = 0 with NaN in C."""
+    og = O.parse(alg_path(name))
+    og = O.compose(og, og)
+    m, n, k = (171, 158, 149) if name == "333" else (150, 161, 139)
+    A, B, C = O.integers(m, k, 48, 0, 3), O.integers(k, n, 48, 1, 3), O.integers(m, n, 48, 2, 3)
+    for alpha, beta in [(1.0, 1.0), (2.0, -3.0), (0.5, 0.0)]:
+        want = O.fmm_dgemm(og, alpha, A, B, beta, C.copy(order="F"))
+        Cn = C.copy(order="F")
+        if beta == 0.0:
+            Cn[:] = np.nan
+        say("two-stage W", name, alpha, beta)
+        assert np.array_equal(run(alg(name, 2, 2, -1), alpha, A, B, beta, Cn), want)
+        a = alg(name, 2, 2, -1)
+        dA, dB, dC = dev(A), dev(B), dev(np.asfortranarray(Cn.T))
+        F.check(F.lib().fmmhip_dgemm_fmm(a.h, 0, m, n, k, alpha, dA.ptr, 1, m, dB.ptr, 1, k, beta, dC.ptr, n, 1,
+                                         None), "dgemm_fmm row-major C")
+        F.sync()
+        assert np.array_equal(dC.to_numpy().T, want)
