@@ -219,3 +219,20 @@ def test_two_stage_w_exact_integers_row_major_c_and_nan(name):
                                          None), "dgemm_fmm row-major C")
         F.sync()
         assert np.array_equal(dC.to_numpy().T, want)
+
+
+@pytest.mark.parametrize("pack", [1, 2])
+def test_two_stage_paths_fp32_operands_exact(pack):
+    """FP32 operands through the two-stage summation (both pre-summed
+    modes) and the two-stage W stage (materialised), 333 at two levels:
+    small integers keep every sum exact in FP32, so C equals the oracle's
+    FMM bit for bit."""
+    og = O.parse(alg_path("333"))
+    og = O.compose(og, og)
+    m, n, k = 170, 163, 155
+    A, B, C = O.integers(m, k, 49, 0, 2), O.integers(k, n, 49, 1, 2), O.integers(m, n, 49, 2, 2)
+    want = O.fmm_dgemm(og, 1.0, A, B, 1.0, C.copy(order="F")).astype(np.float32)
+    A32, B32, C32 = (np.asfortranarray(X.astype(np.float32)) for X in (A, B, C))
+    got = run(alg("333", 2, pack, -1), 1.0, A32, B32, 1.0, C32)
+    assert got.dtype == np.float32
+    assert np.array_equal(got, want)
