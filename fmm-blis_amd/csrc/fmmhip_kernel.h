@@ -16,6 +16,9 @@
 //   W epilogue   plugin/ref_kernels/bli_gemm_fmm_ref.c:101-120 -- each
 //                accumulator tile is added into every C part with a nonzero
 //                W coefficient, clipped to the part; no M_r is stored
+//                (the products-materialised mode passes one W entry per
+//                product instead, the product's slot: a plain store, W
+//                applied afterwards by fmmhip_wsum.hip)
 //   R loop       frame/base/bli_strassen_ab.c:248-294 -- inside the kernel
 //
 // Work decomposition: one workgroup per BM x BN tile position of the leaf,

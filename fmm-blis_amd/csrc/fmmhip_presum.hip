@@ -12,6 +12,11 @@
 // at S + r*ldo*Ls, so the GEMM reads it with the operand's original
 // orientation and 16-byte pairs.  Consecutive threads walk c: reads and
 // writes coalesce.
+//
+// Two-level tables with many parts per operand (333, 444) form the sums in
+// two stages (presum_stage_kernel): the inner level's sums of every outer
+// block's parts, then the outer level's sums of those, each stage over at
+// most 16 parts.
 #include <hip/hip_runtime.h>
 
 #include "fmm_internal.h"
