@@ -819,7 +819,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     const size_t offB = (bytesA + 255) & ~(size_t)255;
     const size_t ws_bytes = offB + esz * wb.elems(alg->R) + 256;
     const int mode = resolve_pack(alg, kind, g, pack, split, esz, ws_bytes,
-                                  (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0, !src);
+                                  (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0, !src || src->can_mat());
     const bool wmat = mode == 2;
     const bool presum = mode >= 1;
 
@@ -827,7 +827,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     memset(&dv, 0, sizeof(dv));
     int rc;
     if (src) {
-        rc = src->get(alg, g, presum, a_mn, b_mn, st, &dv);
+        rc = src->get(alg, g, presum, wmat, a_mn, b_mn, st, &dv);
     } else {
         DevDesc* d = nullptr;
         rc = get_desc(alg, g, presum, wmat, a_mn, b_mn, st, &d);
@@ -1175,16 +1175,23 @@ size_t desc_bound(fmmhip_alg_t alg)
 {
     const long fused = alg->nnz_u + alg->nnz_v + alg->nnz_w;
     const long pres = 2L * alg->R + alg->nnz_w + alg->mt * alg->kt + alg->kt * alg->nt;
-    return ((list_bytes(alg->R, fused) + 255) & ~(size_t)255) + list_bytes(alg->R, pres) + 256;
+    const long mat = 3L * alg->R + alg->mt * alg->nt + alg->nnz_w;
+    return ((list_bytes(alg->R, fused) + 255) & ~(size_t)255) + ((list_bytes(alg->R, pres) + 255) & ~(size_t)255) +
+           list_bytes(alg->R, mat) + 256;
 }
 
-int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, void* host, const void* dev,
-              size_t cap, size_t* used, DescView* v)
+int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool mat, bool a_mn, bool b_mn, void* host,
+              const void* dev, size_t cap, size_t* used, DescView* v)
 {
-    DevList fused, pres;
+    DevList fused, pres, matl;
+    mat = mat && presum;
     int rc = fmm_build_entries(alg, &g, &fused.ptr_h, &fused.ent_h, &fused.nent);
     if (!rc && presum) rc = build_presum_list(alg, g, fused, a_mn, b_mn, pres);
-    size_t need = ((list_bytes(alg->R, fused.nent) + 255) & ~(size_t)255) + (presum ? list_bytes(alg->R, pres.nent) : 0);
+    if (!rc && mat) rc = build_mat_list(alg, g, fused, pres, matl);
+    const size_t off2 = (list_bytes(alg->R, fused.nent) + 255) & ~(size_t)255;
+    const size_t off3 = presum ? off2 + ((list_bytes(alg->R, pres.nent) + 255) & ~(size_t)255) : off2;
+    size_t need = presum ? off2 + list_bytes(alg->R, pres.nent) : off2;
+    if (mat) need = off3 + list_bytes(alg->R, matl.nent);
     if (!rc && need > cap) rc = FMMHIP_ERR_INVALID;
     if (!rc) {
         char* h = (char*)host;
@@ -1202,10 +1209,19 @@ int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, boo
             v->presum_ptr = nullptr;
             v->presum_ent = nullptr;
         }
+        if (mat) {
+            const size_t e3 = pack_list(matl, alg->R, h, off3);
+            v->mat_ptr = (const int*)(d + off3);
+            v->mat_ent = (const fmm_entry_t*)(d + off3 + e3);
+        } else {
+            v->mat_ptr = nullptr;
+            v->mat_ent = nullptr;
+        }
         *used = need;
     }
     free(fused.ptr_h); free(fused.ent_h);
     free(pres.ptr_h); free(pres.ent_h);
+    free(matl.ptr_h); free(matl.ent_h);
     return rc;
 }
 

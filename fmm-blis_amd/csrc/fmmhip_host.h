@@ -77,8 +77,11 @@ struct DescView {
 // Where prepare_call takes the lists from; without one, from the
 // algorithm's per-geometry cache.
 struct DescSource {
-    virtual int get(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, hipStream_t st,
-                    DescView* v) = 0;
+    // mat: also the products-materialised list (only asked of sources whose
+    // can_mat() is true)
+    virtual int get(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool mat, bool a_mn, bool b_mn,
+                    hipStream_t st, DescView* v) = 0;
+    virtual bool can_mat() const { return false; }
     virtual ~DescSource() {}
 };
 
@@ -86,8 +89,8 @@ struct DescSource {
 // them into `host` in their device layout with pointers bound to `dev`
 // (the caller copies host -> dev before the launch).
 size_t desc_bound(fmmhip_alg_t alg);
-int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, void* host, const void* dev,
-              size_t cap, size_t* used, DescView* v);
+int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool mat, bool a_mn, bool b_mn, void* host,
+              const void* dev, size_t cap, size_t* used, DescView* v);
 
 // Resolves one call (pack: 0 fused, 1 pre-summed, 2 pre-summed with the
 // products materialised, -1 automatic; split: product groups per tile, 1,

@@ -117,11 +117,12 @@ struct PackSource : DescSource {
     void* host;
     const void* dev;
     size_t cap, used = 0;
-    int get(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool a_mn, bool b_mn, hipStream_t,
+    int get(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool mat, bool a_mn, bool b_mn, hipStream_t,
             DescView* v) override
     {
-        return desc_pack(alg, g, presum, a_mn, b_mn, host, dev, cap, &used, v);
+        return desc_pack(alg, g, presum, mat, a_mn, b_mn, host, dev, cap, &used, v);
     }
+    bool can_mat() const override { return true; }
 };
 
 }  // namespace
@@ -200,7 +201,7 @@ bool fmmhip_server_s::try_admit(Req& q, Slot& sl, int si)
     src.cap = list_cap;
     Prepared P;
     int rc = prepare_call(alg, r.is_double ? Kind::D64 : Kind::S32, r.m, r.n, r.k, r.alpha, r.A, 1, r.lda, r.B, 1,
-                          r.ldb, r.beta, C, 1, ldc, q.cand > 0 ? 1 : 0, alg->split, sl.st, &P, &src);
+                          r.ldb, r.beta, C, 1, ldc, q.cand > 0 ? 2 : 0, alg->split, sl.st, &P, &src);
     if (rc) {
         arena.release(off1, b1);
         finish(q, rc);

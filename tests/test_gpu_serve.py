@@ -54,8 +54,9 @@ def fetch(ptr, m, n, ld, dtype):
 
 def alone(cand, dt, m, n, k, alpha, dA, dB, beta, dC):
     """The single-call equivalent of a served request: the server's FMM
-    candidates run pre-summed and without product split; the classical one
-    keeps its per-call k-split choice."""
+    candidates run with the products materialised, which gives the same C
+    as pre-summed without product split; the classical one keeps its
+    per-call k-split choice."""
     alg = F.Alg(classical=True) if cand == 0 else F.Alg([F.strassen()] * cand)
     if cand > 0:
         alg.set_pack(1)
@@ -117,12 +118,13 @@ def test_serve_user_c_with_beta():
 
 
 def test_serve_budget_admission_and_overlap():
-    """12 requests of 1024^3 FP64 with two-level Strassen need about 60 MiB
-    each (output, lists, workspace); a 160 MiB budget holds two at a time."""
+    """12 requests of 1024^3 FP64 with two-level Strassen need about 85 MB
+    each (output 8 MB, lists, workspace: the pre-summed leaves 51 MB and the
+    49 product slots 26 MB); a 200 MiB budget holds two at a time."""
     m = n = k = 1024
     A, B = make_inputs("f8", m, n, k, 40)
     dA, dB = dev(A), dev(B)
-    budget = 160 << 20
+    budget = 200 << 20
     srv = F.Server(budget, slots=4)
     ids = [srv.submit(m, n, k, 1.0, dA, dB, cand=2, keep=0) for _ in range(12)]
     srv.drain()
