@@ -15,7 +15,8 @@ from conftest import ALG_NAMES, alg_path
 
 HEADERS = {"222": (2, 2, 2, 7), "232": (2, 3, 2, 11), "322": (3, 2, 2, 11),
            "333": (3, 3, 3, 23), "444": (4, 4, 4, 49)}
-# nonzeros of U, V, W per file (counted in the survey of the reference)
+# nonzeros of U, V, W per file (counted by hand from the reference's
+# coefficient files, independently of the parser under test)
 NNZ = {"222": (12, 12, 12), "232": (16, 16, 16), "322": (16, 17, 17),
        "333": (59, 53, 53), "444": (194, 194, 194)}
 
