@@ -99,7 +99,9 @@ def run_batch(args, F):
     """The batching form: one captured graph over all problems with the
     tuner's choices, beside the same graph with every problem classical and
     beside one classical call per problem (no graph)."""
-    shapes = batch_shapes(args.batch_count, args.seed)
+    uniform = args.batch_uniform > 0
+    shapes = [("f8", args.batch_uniform, args.batch_uniform, args.batch_uniform)] * args.batch_count if uniform \
+        else batch_shapes(args.batch_count, args.seed)
     mats = []
     for i, (dt, m, n, k) in enumerate(shapes):
         A = F.DeviceMatrix(m, k, dtype=dt).fill_uniform(args.seed, 3 * i)
@@ -165,16 +167,19 @@ def run_batch(args, F):
     peak = (leaf64 + leaf32) / t_peak / 1e12
     eff = flops / (ms * 1e-3) / 1e9
     out = {
-        "metric": "effective GFLOPS (2mnk/t) of a 512-GEMM uneven FP64/FP32 batch, tuned FMM vs classical",
+        "metric": ("effective GFLOPS (2mnk/t) of a uniform FP64 batch, tuned FMM vs classical" if uniform else
+                   "effective GFLOPS (2mnk/t) of a 512-GEMM uneven FP64/FP32 batch, tuned FMM vs classical"),
         "value": round(eff, 1),
         "unit": "GFLOPS",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+f32",
+        "dtype": "f64" if uniform else "f64+f32",
         "data": f"synthetic (SplitMix64 uniform[-1,1], seed {args.seed}, device-resident)",
         "config": {
-            "workload": f"batch of {len(shapes)} GEMMs C += A*B, m,n,k log-uniform [256,8192] multiples of 64, "
-                        f"first half FP64, rest FP32, one captured HIP graph",
+            "workload": (f"batch of {len(shapes)} FP64 GEMMs C += A*B, m=n=k={args.batch_uniform}, "
+                         f"one captured HIP graph" if uniform else
+                         f"batch of {len(shapes)} GEMMs C += A*B, m,n,k log-uniform [256,8192] multiples of 64, "
+                         f"first half FP64, rest FP32, one captured HIP graph"),
             "problems": len(shapes), "total_flops": flops, "fp64_flops": flops64,
             "choices_classical_1level_2level": choices,
             "items_f64": info["items_f64"], "items_f32": info["items_f32"],
@@ -428,6 +433,8 @@ def main():
     ap.add_argument("--k", type=int, default=0, help="sweep/single: k if not --size (rank-k shapes)")
     ap.add_argument("--no-bf16x3", action="store_true", help="single: skip the BF16x3 line beside the FP64 one")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-uniform", type=int, default=0,
+                    help="batch form: every problem FP64 of this size cubed (default: the uneven 512-GEMM mix)")
     ap.add_argument("--rate", type=float, default=500.0, help="serve: arrivals per second")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the timed stream")
     ap.add_argument("--slots", type=int, default=8, help="serve: streams (requests in flight)")
