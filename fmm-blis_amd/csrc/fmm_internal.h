@@ -130,6 +130,9 @@ int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
  * lie at `parts[p].off` from the stage's input, sum r is written at
  * s_off + r*pstride from the stage's output. */
 #define FMMHIP_STAGE_PARTS 16
+/* Most products per level of the two-stage summation (its kernel holds a
+ * level's term list, at most 16 terms per product, in LDS). */
+#define FMMHIP_STAGE_MAX_R 64
 typedef struct fmmhip_stage_term_s {
     double coef;
     int32_t part;               /* < np */

@@ -258,7 +258,8 @@ bool kron_operand(const struct fmmhip_alg_s* alg, bool is_a)
     const int np = is_a ? alg->mt * alg->kt : alg->kt * alg->nt;
     const int np0 = is_a ? alg->lv_m[0] * alg->lv_k[0] : alg->lv_k[0] * alg->lv_n[0];
     const int np1 = is_a ? alg->lv_m[1] * alg->lv_k[1] : alg->lv_k[1] * alg->lv_n[1];
-    return np >= kKronMinParts && np0 <= FMMHIP_STAGE_PARTS && np1 <= FMMHIP_STAGE_PARTS;
+    return np >= kKronMinParts && np0 <= FMMHIP_STAGE_PARTS && np1 <= FMMHIP_STAGE_PARTS &&
+           alg->lv_R[0] <= FMMHIP_STAGE_MAX_R && alg->lv_R[1] <= FMMHIP_STAGE_MAX_R;
 }
 
 // Host image of one operand's two-stage tables.  Composed part (row

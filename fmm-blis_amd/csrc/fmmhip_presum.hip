@@ -219,36 +219,59 @@ __global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ 
     constexpr int NP = FMMHIP_STAGE_PARTS;
     const int si = (int)blockIdx.x / bps;
     const fmmhip_stage_seg_t sg = segs[si];
-    __shared__ long p_off[NP];
-    __shared__ int p_c[NP], p_s[NP];
-    parts_meta<NP, TB>(sg.parts, sg.np, swap, p_off, p_c, p_s);
+    // The segment's part descriptors and term list into LDS, read back
+    // below as uniform (scalar) values: read from global memory in the
+    // loops, each would be a vector load whose wait also waits for every
+    // store issued before it.
+    __shared__ long p_off_l[NP];
+    __shared__ int p_c_l[NP], p_s_l[NP];
+    __shared__ int tp_l[FMMHIP_STAGE_MAX_R + 1];
+    __shared__ fmmhip_stage_term_t tm_l[FMMHIP_STAGE_PARTS * FMMHIP_STAGE_MAX_R];
+    parts_meta<NP, TB>(sg.parts, sg.np, swap, p_off_l, p_c_l, p_s_l);
+    const int nterm = sg.tptr[sg.R];
+    for (int i = threadIdx.x; i <= sg.R; i += TB) tp_l[i] = sg.tptr[i];
+    for (int i = threadIdx.x; i < nterm; i += TB) tm_l[i] = sg.terms[i];
     __syncthreads();
     const long nbc = (Lc + 2 * TB - 1) / (2 * TB);
     T* __restrict__ So = S + sg.s_off;
     for (long u = (long)blockIdx.x - (long)si * bps; u < nbc * Ls; u += bps) {
         const int s = (int)(u / nbc);
         const int c0 = (int)(u - (long)s * nbc) * 2 * TB + 2 * (int)threadIdx.x;
+        // s is uniform too, so each part's row start is a scalar base
+        // address and only the offset along c is per lane; every address is
+        // clamped into its part (or to X[0] for an empty part), all 32 loads
+        // are in flight together, and elements outside the part are replaced
+        // by zero afterwards.
+        double r0[NP], r1[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const int p = min(j, sg.np - 1);
+            const int ec = __builtin_amdgcn_readfirstlane(p_c_l[p]), es = __builtin_amdgcn_readfirstlane(p_s_l[p]);
+            const long off = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(p_off_l[p] >> 32)) << 32) |
+                                    (unsigned)__builtin_amdgcn_readfirstlane((int)p_off_l[p]));
+            const bool live = (ec > 0) & (es > 0);
+            const T* __restrict__ base = X + (live ? off + (long)min(s, es - 1) * ld : 0);
+            r0[j] = (double)base[live ? min(c0, ec - 1) : 0];
+            r1[j] = (double)base[live ? min(c0 + 1, ec - 1) : 0];
+        }
         double x0[NP], x1[NP];
 #pragma unroll
         for (int j = 0; j < NP; j++) {
             const int p = min(j, sg.np - 1);
-            const int ec = p_c[p], es = p_s[p];
-            const bool live = (ec > 0) & (es > 0);
-            const long row = p_off[p] + (long)min(s, es - 1) * ld;
-            const double v0 = (double)X[live ? row + min(c0, ec - 1) : 0];
-            const double v1 = (double)X[live ? row + min(c0 + 1, ec - 1) : 0];
+            const int ec = __builtin_amdgcn_readfirstlane(p_c_l[p]), es = __builtin_amdgcn_readfirstlane(p_s_l[p]);
             const bool in = (s < es) & (j < sg.np);
-            x0[j] = (in & (c0 < ec)) ? v0 : 0.0;
-            x1[j] = (in & (c0 + 1 < ec)) ? v1 : 0.0;
+            x0[j] = (in & (c0 < ec)) ? r0[j] : 0.0;
+            x1[j] = (in & (c0 + 1 < ec)) ? r1[j] : 0.0;
         }
         if (c0 < Lc) {
             T* out = So + c0 + (long)s * ldo;
             const bool two = c0 + 1 < Lc;
             for (int r = 0; r < sg.R; r++) {
                 double v0 = 0.0, v1 = 0.0;
-                for (int t = sg.tptr[r]; t < sg.tptr[r + 1]; t++) {
-                    const fmmhip_stage_term_t tm = sg.terms[t];
-                    switch (tm.part) {
+                const int t1 = __builtin_amdgcn_readfirstlane(tp_l[r + 1]);
+                for (int t = __builtin_amdgcn_readfirstlane(tp_l[r]); t < t1; t++) {
+                    const fmmhip_stage_term_t tm = tm_l[t];
+                    switch (__builtin_amdgcn_readfirstlane(tm.part)) {
 #define STAGE_TERM(K)                                                                                         \
     case K:                                                                                                   \
         v0 = fma(tm.coef, x0[K], v0);                                                                         \
