@@ -26,6 +26,56 @@
 siz_t FMM_BLIS_PACK_UKR = 0;
 siz_t FMM_BLIS_GEMM_UKR = 0;
 
+/* ---- kernel registry ---------------------------------------------------- */
+
+/* The counterpart of the BLIS gks kernel slots and the per-configuration
+ * context (blis/frame/base/bli_gks.c bli_gks_register_ukr,
+ * bli_cntx_set_ukr / bli_cntx_get_ukr_dt): slot ids are handed out in
+ * order, and the one configuration (the gfx950 device) holds, per slot, a
+ * kernel per real datatype (index 0 float, 1 double; complex slots stay
+ * empty, as the library has no complex kernels). */
+#define FMM_UKR_SLOTS 8
+static void* g_ukr[FMM_UKR_SLOTS][2];
+static siz_t g_ukr_count = 0;
+static pthread_mutex_t g_ukr_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static err_t register_ukr(siz_t* id)
+{
+    pthread_mutex_lock(&g_ukr_mu);
+    err_t e = BLIS_FAILURE;
+    if (g_ukr_count + 1 < FMM_UKR_SLOTS) { *id = ++g_ukr_count; e = BLIS_SUCCESS; }
+    pthread_mutex_unlock(&g_ukr_mu);
+    return e;
+}
+
+static void set_ukr(siz_t id, void* s_fn, void* d_fn)
+{
+    pthread_mutex_lock(&g_ukr_mu);
+    if (id > 0 && id < FMM_UKR_SLOTS) { g_ukr[id][0] = s_fn; g_ukr[id][1] = d_fn; }
+    pthread_mutex_unlock(&g_ukr_mu);
+}
+
+void* fmm_blis_get_ukr_dt(num_t dt, siz_t ukr_id)
+{
+    if (bli_plugin_register_fmm_blis() != BLIS_SUCCESS) return NULL;
+    if (ukr_id == 0 || ukr_id >= FMM_UKR_SLOTS) return NULL;
+    const int t = dt == BLIS_DOUBLE ? 1 : dt == BLIS_FLOAT ? 0 : -1;
+    if (t < 0) return NULL;
+    pthread_mutex_lock(&g_ukr_mu);
+    void* f = g_ukr[ukr_id][t];
+    pthread_mutex_unlock(&g_ukr_mu);
+    return f;
+}
+
+/* Per-configuration init (plugin/ref_kernels/bli_plugin_init_ref.c:61-71):
+ * the pack slot gets the summation stage, the gemm slot the MFMA leaf
+ * products with the W epilogue, for float and double. */
+void bli_plugin_init_fmm_blis_gfx950_ref(void)
+{
+    set_ukr(FMM_BLIS_PACK_UKR, (void*)fmmhip_spack_fmm, (void*)fmmhip_dpack_fmm);
+    set_ukr(FMM_BLIS_GEMM_UKR, (void*)fmmhip_sgemm_fmm_packed, (void*)fmmhip_dgemm_fmm_packed);
+}
+
 /* ---- registration ------------------------------------------------------- */
 
 static pthread_once_t g_reg_once = PTHREAD_ONCE_INIT;
@@ -34,11 +84,12 @@ static err_t g_reg_err = BLIS_FAILURE;
 static void do_register(void)
 {
     if (fmmhip_init() != FMMHIP_SUCCESS) { g_reg_err = BLIS_FAILURE; return; }
-    /* Two kernel slots, as the reference allocates one for the pack and one
-     * for the gemm kernel (bli_plugin_register.c:47-48).  Here both stages
-     * are one fused device kernel; the ids identify it to callers. */
-    FMM_BLIS_PACK_UKR = 1;
-    FMM_BLIS_GEMM_UKR = 2;
+    /* two kernel slots, then the context init of every enabled
+     * configuration (bli_plugin_register.c:47-61); here one, the device */
+    err_t e = register_ukr(&FMM_BLIS_PACK_UKR);
+    if (e == BLIS_SUCCESS) e = register_ukr(&FMM_BLIS_GEMM_UKR);
+    if (e != BLIS_SUCCESS) { g_reg_err = e; return; }
+    bli_plugin_init_fmm_blis_gfx950_ref();
     g_reg_err = BLIS_SUCCESS;
 }
 
@@ -141,6 +192,7 @@ void fmm_obj_scalar(double* storage, obj_t* o)
 typedef struct {
     fmmhip_alg_t alg;
     int nlevels;
+    int pins;               /* calls using alg now: a pinned slot is never evicted */
     fmm_t lv[4];            /* deep copies */
 } alg_slot_t;
 
@@ -170,9 +222,13 @@ static void free_slot(alg_slot_t* s)
     memset(s, 0, sizeof(*s));
 }
 
-/* Returns a cached composed algorithm for these levels (or creates it). */
-static int lookup_alg(const fmm_t* algs, int nlevels, fmmhip_alg_t* out)
+/* Returns a cached composed algorithm for these levels (or creates it),
+ * pinned for the caller until release_alg.  Eviction (round robin) skips
+ * pinned slots; when every slot is pinned the new algorithm is not cached
+ * and *owned = 1 (release_alg destroys it). */
+static int lookup_alg(const fmm_t* algs, int nlevels, fmmhip_alg_t* out, int* owned)
 {
+    *owned = 0;
     for (int l = 0; l < nlevels; l++)
         if (algs[l].m_tilde <= 0 || algs[l].k_tilde <= 0 || algs[l].n_tilde <= 0 || algs[l].R <= 0 ||
             !algs[l].U || !algs[l].V || !algs[l].W)
@@ -183,16 +239,35 @@ static int lookup_alg(const fmm_t* algs, int nlevels, fmmhip_alg_t* out)
         if (s->nlevels != nlevels) continue;
         int same = 1;
         for (int l = 0; l < nlevels && same; l++) same = same_fmm(&s->lv[l], &algs[l]);
-        if (same) { *out = s->alg; pthread_mutex_unlock(&g_alg_mu); return FMMHIP_SUCCESS; }
+        if (same) {
+            s->pins++;
+            *out = s->alg;
+            pthread_mutex_unlock(&g_alg_mu);
+            return FMMHIP_SUCCESS;
+        }
     }
     fmmhip_alg_t alg;
     int rc = fmmhip_alg_create(&alg, algs, nlevels);
     if (rc) { pthread_mutex_unlock(&g_alg_mu); return rc; }
-    alg_slot_t* s;
-    if (g_nslots < ALG_SLOTS) s = &g_slots[g_nslots++];
-    else { s = &g_slots[g_next_victim]; g_next_victim = (g_next_victim + 1) % ALG_SLOTS; free_slot(s); }
+    alg_slot_t* s = NULL;
+    if (g_nslots < ALG_SLOTS) {
+        s = &g_slots[g_nslots++];
+    } else {
+        for (int t = 0; t < ALG_SLOTS && !s; t++) {
+            alg_slot_t* c = &g_slots[(g_next_victim + t) % ALG_SLOTS];
+            if (c->pins == 0) { s = c; g_next_victim = (g_next_victim + t + 1) % ALG_SLOTS; }
+        }
+        if (s) free_slot(s);
+    }
+    if (!s) {
+        *owned = 1;
+        *out = alg;
+        pthread_mutex_unlock(&g_alg_mu);
+        return FMMHIP_SUCCESS;
+    }
     s->alg = alg;
     s->nlevels = nlevels;
+    s->pins = 1;
     for (int l = 0; l < nlevels; l++) {
         s->lv[l] = algs[l];
         s->lv[l].U = malloc(tab_bytes(&algs[l], 0)); memcpy(s->lv[l].U, algs[l].U, tab_bytes(&algs[l], 0));
@@ -202,6 +277,28 @@ static int lookup_alg(const fmm_t* algs, int nlevels, fmmhip_alg_t* out)
     *out = alg;
     pthread_mutex_unlock(&g_alg_mu);
     return FMMHIP_SUCCESS;
+}
+
+/* Unpins an algorithm returned by lookup_alg (destroys an uncached one). */
+static void release_alg(fmmhip_alg_t alg, int owned)
+{
+    if (!alg) return;
+    if (owned) { fmmhip_alg_destroy(alg); return; }
+    pthread_mutex_lock(&g_alg_mu);
+    for (int i = 0; i < g_nslots; i++)
+        if (g_slots[i].alg == alg && g_slots[i].pins > 0) { g_slots[i].pins--; break; }
+    pthread_mutex_unlock(&g_alg_mu);
+}
+
+/* Number of cached algorithms and of pinned ones (for tests). */
+void fmm_blis_amd_alg_cache_stats(int* cached, int* pinned)
+{
+    pthread_mutex_lock(&g_alg_mu);
+    int p = 0;
+    for (int i = 0; i < g_nslots; i++) p += g_slots[i].pins > 0;
+    if (cached) *cached = g_nslots;
+    if (pinned) *pinned = p;
+    pthread_mutex_unlock(&g_alg_mu);
 }
 
 /* ---- host staging -------------------------------------------------------- */
@@ -276,13 +373,14 @@ static void fmm_driver(const char* who, obj_t* alpha, obj_t* A, obj_t* B, obj_t*
     /* bli_l3_return_early_if_trivial */
     if (m == 0 || n == 0) return;
 
-    fmmhip_alg_t alg = NULL;
-    if (!classical) {
-        int rc = lookup_alg(algs, nlevels, &alg);
-        if (rc) { fail(who, rc, "invalid algorithm (zero-sized, NULL tables or Brent equations violated)"); return; }
-    }
     const int dbl = vc.dt == BLIS_DOUBLE;
     if (!dbl && leaf != FMMHIP_LEAF_NATIVE) { fail(who, FMMHIP_ERR_UNSUPPORTED, "reduced leaf needs double objects"); return; }
+    fmmhip_alg_t alg = NULL;
+    int alg_owned = 0;
+    if (!classical) {
+        int rc = lookup_alg(algs, nlevels, &alg, &alg_owned);
+        if (rc) { fail(who, rc, "invalid algorithm (zero-sized, NULL tables or Brent equations violated)"); return; }
+    }
 
     int dev_a = va.m && va.n ? fmmhip_pointer_is_device(va.p) : 1;
     int dev_b = vb.m && vb.n ? fmmhip_pointer_is_device(vb.p) : 1;
@@ -332,8 +430,8 @@ static void fmm_driver(const char* who, obj_t* alpha, obj_t* A, obj_t* B, obj_t*
             pc = g_stage[2].p;
         }
     }
-    if (ua.rs != 1 && ua.cs != 1 && k > 0) { rc = FMMHIP_ERR_UNSUPPORTED; goto out; }
-    if (ub.rs != 1 && ub.cs != 1 && k > 0) { rc = FMMHIP_ERR_UNSUPPORTED; goto out; }
+    /* device operands with neither stride 1 (e.g. strided sub-views) are
+     * gathered on the device by the native layer */
 
     if (dbl) {
         if (classical)
@@ -358,6 +456,7 @@ static void fmm_driver(const char* who, obj_t* alpha, obj_t* A, obj_t* B, obj_t*
     }
 out:
     if (locked) pthread_mutex_unlock(&g_stage_mu);
+    release_alg(alg, alg_owned);
     free(tmp_a);
     free(tmp_b);
     if (rc) fail(who, rc, "device computation failed");

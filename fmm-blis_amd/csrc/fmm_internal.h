@@ -40,6 +40,7 @@ struct fmmhip_alg_s {
     int  max_nnz_u, max_nnz_v, max_nnz_w;
     int  shape;                 /* kernel launch shape, -1 = chosen per call */
     int  pack;                  /* 0 fused, 1 pre-summed workspace, -1 = chosen per call */
+    int  last_pack;             /* mode the last FP64/FP32 call took (-1: none yet) */
     int  keep_all;              /* never evict cached descriptors (owners that capture graphs) */
     int  split;                 /* product groups per tile: -1 chosen per call, 1 (none), 2, 4, 8 */
     /* two-level algorithms: the two levels' own tables (level 0 outer),
@@ -67,6 +68,10 @@ void fmm_part_split(long whole, int tilde, long* size, long* off);
  * status. */
 int fmm_build_entries(const struct fmmhip_alg_s* alg, const fmm_geom_t* g,
                       int32_t** ptr_out, fmm_entry_t** ent_out, long* nent_out);
+/* The same lists into caller memory: ptr[3*(R+1)], ent[nnz_u+nnz_v+nnz_w]
+ * (zeroed by the caller), and 2*(mt+kt+nt) longs of `sizes` to work in. */
+void fmm_build_entries_into(const struct fmmhip_alg_s* alg, const fmm_geom_t* g, int32_t* ptr, fmm_entry_t* ent,
+                            long* sizes);
 
 /* HIP layer hooks used by the C files. */
 void fmmhip_alg_release_cache(struct fmmhip_alg_s* alg);
@@ -77,6 +82,13 @@ int fmmhip_scale_s(float* C, long m, long n, long rsC, long csC, float beta, voi
 int fmmhip_pointer_is_device(const void* p);
 /* The library's default stream (created on first use). */
 void* fmmhip_default_stream(void);
+/* Largest workspace (bytes) an automatic pack-mode choice may take: half of
+ * the device's total memory, a property of the device rather than of what
+ * is free at the time of a call. */
+double fmmhip_ws_bound(void);
+/* Y := X (m x n, element (i,j) at X[i*rs + j*cs]) into column-major Y
+ * (ld = m), asynchronous on `stream`. */
+int fmmhip_gather(int is_double, const void* X, long m, long n, long rs, long cs, void* Y, void* stream);
 /* Split-product epilogue helpers (fmmhip_runtime.hip): zero G temporaries,
  * then C := beta*C + sum_g T_g over an m x n matrix with strides (rs, cs);
  * each temporary spans `elems` elements with C's strides. */

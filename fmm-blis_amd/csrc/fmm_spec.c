@@ -316,6 +316,7 @@ int fmmhip_alg_create(fmmhip_alg_t* out, const fmm_t* algs, int nlevels)
     g->shape = -1;
     g->split = -1;
     g->pack = -1;
+    g->last_pack = -1;
     /* start from level 0 and fold the next levels in as inner blocks */
     int cm = algs[0].m_tilde, ck = algs[0].k_tilde, cn = algs[0].n_tilde, cR = algs[0].R;
     int* U = (int*)malloc(sizeof(int) * (size_t)(cm * ck * cR));
@@ -409,17 +410,28 @@ int fmm_build_entries(const struct fmmhip_alg_s* g, const fmm_geom_t* geo,
                       int32_t** ptr_out, fmm_entry_t** ent_out, long* nent_out)
 {
     const int mt = g->mt, kt = g->kt, nt = g->nt, R = g->R;
-    long* szm = (long*)malloc(sizeof(long) * 2 * (size_t)(mt + kt + nt));
-    if (!szm) return FMMHIP_ERR_NOMEM;
+    long* sizes = (long*)malloc(sizeof(long) * 2 * (size_t)(mt + kt + nt));
+    long nent = g->nnz_u + g->nnz_v + g->nnz_w;
+    int32_t* ptr = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
+    fmm_entry_t* ent = (fmm_entry_t*)calloc((size_t)(nent + 1), sizeof(fmm_entry_t));
+    if (!sizes || !ptr || !ent) { free(sizes); free(ptr); free(ent); return FMMHIP_ERR_NOMEM; }
+    fmm_build_entries_into(g, geo, ptr, ent, sizes);
+    free(sizes);
+    *ptr_out = ptr;
+    *ent_out = ent;
+    *nent_out = nent;
+    return FMMHIP_SUCCESS;
+}
+
+void fmm_build_entries_into(const struct fmmhip_alg_s* g, const fmm_geom_t* geo, int32_t* ptr, fmm_entry_t* ent,
+                            long* sizes)
+{
+    const int mt = g->mt, kt = g->kt, nt = g->nt, R = g->R;
+    long* szm = sizes;
     long *ofm = szm + mt, *szk = ofm + mt, *ofk = szk + kt, *szn = ofk + kt, *ofn = szn + nt;
     fmm_part_split(geo->m, mt, szm, ofm);
     fmm_part_split(geo->k, kt, szk, ofk);
     fmm_part_split(geo->n, nt, szn, ofn);
-
-    long nent = g->nnz_u + g->nnz_v + g->nnz_w;
-    int32_t* ptr = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
-    fmm_entry_t* ent = (fmm_entry_t*)calloc((size_t)(nent + 1), sizeof(fmm_entry_t));
-    if (!ptr || !ent) { free(szm); free(ptr); free(ent); return FMMHIP_ERR_NOMEM; }
 
     /* pointers are absolute indices into ent[] */
     for (int r = 0; r <= R; r++) {
@@ -453,9 +465,4 @@ int fmm_build_entries(const struct fmmhip_alg_s* g, const fmm_geom_t* geo,
         ent[e].first = g->w_first[i];
         ent[e].part = s;
     }
-    free(szm);
-    *ptr_out = ptr;
-    *ent_out = ent;
-    *nent_out = nent;
-    return FMMHIP_SUCCESS;
 }

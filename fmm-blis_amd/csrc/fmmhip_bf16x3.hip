@@ -485,13 +485,11 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const size_t oPB = al(planesA), oEA = oPB + al(planesB), oEB = oEA + al(expA);
     const long tiles = (LpA / Cfg3::BM) * (LpB / Cfg3::BN);
     // products materialised (pack 2, or automatic when the slots fit in half
-    // of the free device memory): the kernel stores each product once and
-    // the W stage applies W -- the kernel's W epilogue then only stores
+    // of the device's memory): the kernel stores each product once and the
+    // W stage applies W -- the kernel's W epilogue then only stores (same C
+    // bit for bit either way)
     bool wmat = R > 1 && (alg->pack == 2 || alg->pack < 0);
-    if (wmat && alg->pack < 0) {
-        size_t fr = 0, tot = 0;
-        wmat = hipMemGetInfo(&fr, &tot) == hipSuccess && 8.0 * R * (double)g.lm * (double)g.ln < 0.5 * (double)fr;
-    }
+    if (wmat && alg->pack < 0) wmat = 8.0 * R * (double)g.lm * (double)g.ln < fmmhip_ws_bound();
     MatView mv;
     memset(&mv, 0, sizeof(mv));
     if (wmat && (rc = mat_view(alg, g, st, &mv))) return rc;

@@ -122,43 +122,58 @@ WsLayout ws_layout_b(const fmm_geom_t& g, bool b_mn)
 
 // The pre-summed list: product r reads leaf r of the A and B workspaces
 // with coefficient 1 over the full leaf; its W entries are the fused ones.
-int build_presum_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const DevList& fused, bool a_mn,
-                      bool b_mn, DevList& out)
+// Entries of the pre-summed list: R A leaves, R B leaves, the W entries,
+// then the part tables of A (mt*kt) and B (kt*nt) for the part-staged
+// summation.
+long presum_list_entries(const struct fmmhip_alg_s* alg)
+{
+    return 2L * alg->R + alg->nnz_w + (long)alg->mt * alg->kt + (long)alg->kt * alg->nt;
+}
+
+// The pre-summed list into caller memory (ptr[3*(R+1)], ent zeroed) from
+// the fused list's host arrays.
+void build_presum_list_into(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const int32_t* fptr,
+                            const fmm_entry_t* fent, bool a_mn, bool b_mn, int32_t* ptr, fmm_entry_t* ent)
 {
     const int R = alg->R;
     const WsLayout wa = ws_layout_a(g, a_mn), wb = ws_layout_b(g, b_mn);
-    // entries: R A leaves, R B leaves, the W entries, then the part tables
-    // of A (mt*kt) and B (kt*nt) for the part-staged summation
-    const int npa = alg->mt * alg->kt, npb = alg->kt * alg->nt;
-    out.nent = 2L * R + alg->nnz_w + npa + npb;
-    out.ptr_h = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
-    out.ent_h = (fmm_entry_t*)calloc((size_t)out.nent + 1, sizeof(fmm_entry_t));
-    if (!out.ptr_h || !out.ent_h) return FMMHIP_ERR_NOMEM;
+    const int npa = alg->mt * alg->kt;
     for (int r = 0; r <= R; r++) {
-        out.ptr_h[r] = r;
-        out.ptr_h[(R + 1) + r] = R + r;
-        out.ptr_h[2 * (R + 1) + r] = 2 * R + (fused.ptr_h[2 * (R + 1) + r] - (int32_t)(alg->nnz_u + alg->nnz_v));
+        ptr[r] = r;
+        ptr[(R + 1) + r] = R + r;
+        ptr[2 * (R + 1) + r] = 2 * R + (fptr[2 * (R + 1) + r] - (int32_t)(alg->nnz_u + alg->nnz_v));
     }
     for (int r = 0; r < R; r++) {
-        fmm_entry_t& ea = out.ent_h[r];
+        fmm_entry_t& ea = ent[r];
         ea.off = (int64_t)r * wa.ldo * wa.Ls;
         ea.rows = (int32_t)g.lm;
         ea.cols = (int32_t)g.lk;
         ea.coef = 1.0;
         ea.part = -1;
-        fmm_entry_t& eb = out.ent_h[R + r];
+        fmm_entry_t& eb = ent[R + r];
         eb.off = (int64_t)r * wb.ldo * wb.Ls;
         eb.rows = (int32_t)g.lk;
         eb.cols = (int32_t)g.ln;
         eb.coef = 1.0;
         eb.part = -1;
     }
-    memcpy(out.ent_h + 2 * R, fused.ent_h + alg->nnz_u + alg->nnz_v, sizeof(fmm_entry_t) * (size_t)alg->nnz_w);
+    memcpy(ent + 2 * R, fent + alg->nnz_u + alg->nnz_v, sizeof(fmm_entry_t) * (size_t)alg->nnz_w);
     // part tables (unreferenced parts keep zero extents: nothing is read)
-    fmm_entry_t* pa = out.ent_h + 2 * R + alg->nnz_w;
+    fmm_entry_t* pa = ent + 2 * R + alg->nnz_w;
     fmm_entry_t* pb = pa + npa;
-    for (long e = 0; e < alg->nnz_u; e++) pa[fused.ent_h[e].part] = fused.ent_h[e];
-    for (long e = alg->nnz_u; e < alg->nnz_u + alg->nnz_v; e++) pb[fused.ent_h[e].part] = fused.ent_h[e];
+    for (long e = 0; e < alg->nnz_u; e++) pa[fent[e].part] = fent[e];
+    for (long e = alg->nnz_u; e < alg->nnz_u + alg->nnz_v; e++) pb[fent[e].part] = fent[e];
+}
+
+int build_presum_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const DevList& fused, bool a_mn,
+                      bool b_mn, DevList& out)
+{
+    const int R = alg->R;
+    out.nent = presum_list_entries(alg);
+    out.ptr_h = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(R + 1));
+    out.ent_h = (fmm_entry_t*)calloc((size_t)out.nent + 1, sizeof(fmm_entry_t));
+    if (!out.ptr_h || !out.ent_h) return FMMHIP_ERR_NOMEM;
+    build_presum_list_into(alg, g, fused.ptr_h, fused.ent_h, a_mn, b_mn, out.ptr_h, out.ent_h);
     return FMMHIP_SUCCESS;
 }
 
@@ -599,19 +614,22 @@ int get_desc(fmmhip_alg_t alg, const fmm_geom_t& g, bool want_presum, bool want_
 
 // Per-stream workspace for the pre-summed mode (grow-only; a stream's
 // kernels are ordered, so one buffer per stream is race-free).
+// Tag 0: the call's workspace; 1, 2: the gathered copies of general-strided
+// A and B.
 struct Workspace {
     hipStream_t stream;
+    int tag;
     void* p;
     size_t cap;
 };
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
 
-void* workspace_get(hipStream_t st, size_t bytes)
+void* workspace_get(hipStream_t st, size_t bytes, int tag = 0)
 {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     for (Workspace& w : g_ws) {
-        if (w.stream != st) continue;
+        if (w.stream != st || w.tag != tag) continue;
         if (w.cap >= bytes) return w.p;
         if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
         (void)hipFree(w.p);
@@ -623,7 +641,7 @@ void* workspace_get(hipStream_t st, size_t bytes)
     }
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
-    g_ws.push_back(Workspace{st, p, bytes});
+    g_ws.push_back(Workspace{st, tag, p, bytes});
     return p;
 }
 
@@ -665,8 +683,6 @@ void tm_record(hipEvent_t* ev, hipStream_t st)
 // Largest part count of the part-staged summation kernel.
 constexpr int kPartStagedMax = 256;
 
-// Largest workspace the automatic choice takes for the pre-summed mode.
-constexpr double kWsFractionOfFree = 0.5;
 
 }  // namespace
 
@@ -758,8 +774,11 @@ bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, si
 // products materialised.  Asked for (pack >= 0; materialising needs
 // several products and the one-problem path, whose lists it builds), or
 // automatic: pre-summed for multi-part algorithms when its workspace fits
-// in half of the free device memory, materialised as well when the slots
-// fit too and the time model expects it to be faster (presum_wins).
+// the bound (half of the device's total memory: the choice, and with it the
+// rounding of the result, does not depend on what is free at the time of
+// the call), materialised as well when the slots fit too and the time
+// model expects it to be faster (presum_wins).  prepare_call checks the
+// chosen mode's whole workspace against the bound afterwards.
 int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int split, size_t esz, size_t ws_bytes,
                  double c_span, bool one_problem)
 {
@@ -767,15 +786,15 @@ int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int
     const bool can_mat = alg->R > 1 && one_problem && g.lm <= (1L << 23) && g.ln <= (1L << 23);
     if (pack >= 0) return pack == 2 ? (can_mat ? 2 : 1) : pack;
     if (alg->max_nnz_u <= 1 && alg->max_nnz_v <= 1) return 0;
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess || !(ws_bytes < kWsFractionOfFree * (double)fr)) return 0;
+    const double bound = fmmhip_ws_bound();
+    if (!((double)ws_bytes < bound)) return 0;
     if (!can_mat) return 1;
     const size_t slots = esz * (size_t)g.lm * (size_t)g.ln * (size_t)alg->R;
-    if (!(ws_bytes + slots < kWsFractionOfFree * (double)fr)) return 1;
+    if (!((double)(ws_bytes + slots) < bound)) return 1;
     return presum_wins(alg, kind, g, split, esz, c_span) ? 1 : 2;
 }
 
-int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
+int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
                  int split, hipStream_t st, Prepared* P, DescSource* src)
 {
@@ -823,6 +842,7 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
                                   (double)(m - 1) * rsC + (double)(n - 1) * csC + 1.0, !src || src->can_mat());
     const bool wmat = mode == 2;
     const bool presum = mode >= 1;
+    P->mode = mode;
 
     DescView dv;
     memset(&dv, 0, sizeof(dv));
@@ -999,6 +1019,21 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     return FMMHIP_SUCCESS;
 }
 
+int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
+                 long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
+                 int split, hipStream_t st, Prepared* P, DescSource* src)
+{
+    int rc = prepare_once(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, pack, split, st, P,
+                          src);
+    // automatic mode: the chosen mode's whole workspace (summation leaves,
+    // two-stage leaves, slots, Y leaves, split temporaries) must fit the
+    // bound, else the next simpler mode is taken
+    while (!rc && pack < 0 && P->mode > 0 && (double)P->ws_bytes > fmmhip_ws_bound())
+        rc = prepare_once(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, P->mode - 1, split,
+                          st, P, src);
+    return rc;
+}
+
 void bind_workspace(Prepared* P, void* ws)
 {
     char* base = (char*)ws;
@@ -1023,16 +1058,17 @@ void bind_workspace(Prepared* P, void* ws)
 
 // Launches the summation stage (if any) and the MFMA kernel of a prepared,
 // workspace-bound call on its stream.
-int launch_prepared(const Prepared* P)
+int launch_prepared(const Prepared* P, int stages)
 {
     hipStream_t st = P->stream;
     if (P->nothing) return FMMHIP_SUCCESS;
     if (P->scale_only) {
+        if (!(stages & 2)) return FMMHIP_SUCCESS;
         if (P->kind == Kind::S32) return fmmhip_scale_s((float*)P->C, P->m, P->n, P->rsC, P->csC, (float)P->beta, st);
         return fmmhip_scale_d((double*)P->C, P->m, P->n, P->rsC, P->csC, P->beta, st);
     }
-    bool timed;
-    {
+    bool timed = false;
+    if (stages == 3 && !P->untimed) {
         std::lock_guard<std::mutex> lk(g_tm_mu);
         timed = g_tm_on;
     }
@@ -1041,11 +1077,11 @@ int launch_prepared(const Prepared* P)
     int rc = FMMHIP_SUCCESS;
     const bool dbl = P->kind != Kind::S32;
     const bool split = P->ka.groups > 1 && !P->wmat;
-    if (split) {
+    if (split && (stages & 2)) {
         rc = fmmhip_split_zero(dbl, P->ka.tmp, P->ka.tmp_elems, P->ka.groups, st);
         if (rc) return rc;
     }
-    if (P->presum) {
+    if (P->presum && (stages & 1)) {
         // per operand: the two-stage form, the part-staged one-pass form, or
         // the entry-driven one (part counts beyond the part-staged kernel's)
         auto sum_operand = [&](const fmmhip_presum_seg_t& sg, bool two_stage, const fmmhip_stage_seg_t* s1, int n1,
@@ -1068,6 +1104,7 @@ int launch_prepared(const Prepared* P)
         if (rc) return rc;
     }
     if (timed) tm_record(&tc.e1, st);
+    if (!(stages & 2)) return FMMHIP_SUCCESS;
     switch (P->kind) {
     case Kind::D64: rc = fmmk_launch_d64(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
     case Kind::D_F32LEAF:
@@ -1101,6 +1138,25 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
     // from the workspace lookup to the last launch: a call from another host
     // thread on the same stream cannot regrow the workspace in between
     std::unique_lock<std::mutex> lk = enqueue_lock();
+    // general-strided operands (neither stride 1, e.g. a strided sub-view):
+    // gathered column-major into the stream's gather buffers by one
+    // bandwidth pass, and the call proceeds on the copies
+    const size_t esz = kind == Kind::S32 ? 4 : 8;
+    const bool work = m > 0 && n > 0 && k > 0 && alpha != 0.0 && A && B;
+    if (work && rsA != 1 && csA != 1) {
+        if (rsA < 0 || csA < 0) return FMMHIP_ERR_UNSUPPORTED;
+        void* g = workspace_get(st, esz * (size_t)m * (size_t)k, 1);
+        if (!g) return FMMHIP_ERR_NOMEM;
+        if ((rc = fmmhip_gather(kind != Kind::S32, A, m, k, rsA, csA, g, st))) return rc;
+        A = g; rsA = 1; csA = m;
+    }
+    if (work && rsB != 1 && csB != 1) {
+        if (rsB < 0 || csB < 0) return FMMHIP_ERR_UNSUPPORTED;
+        void* g = workspace_get(st, esz * (size_t)k * (size_t)n, 2);
+        if (!g) return FMMHIP_ERR_NOMEM;
+        if ((rc = fmmhip_gather(kind != Kind::S32, B, k, n, rsB, csB, g, st))) return rc;
+        B = g; rsB = 1; csB = k;
+    }
     const int pack = alg->pack;
     rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, pack, alg->split, st,
                       &P);
@@ -1110,13 +1166,16 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
         if (!ws) {
             if (pack >= 1 || alg->split > 1) return FMMHIP_ERR_NOMEM;
             // automatic choices fall back to the fused mode without split
+            // (reported by fmmhip_alg_last_pack)
             rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, 0, 1, st, &P);
             if (rc) return rc;
         } else {
             bind_workspace(&P, ws);
         }
     }
-    return launch_prepared(&P);
+    rc = launch_prepared(&P);
+    if (!rc && !P.nothing && !P.scale_only) alg->last_pack = P.mode;
+    return rc;
 }
 
 int fused_desc(fmmhip_alg_t alg, const fmm_geom_t& g, hipStream_t st, const int** dptr, const fmm_entry_t** dent)
@@ -1226,6 +1285,37 @@ int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool mat, bool
     return rc;
 }
 
+int desc_pack_presum_into(fmmhip_alg_t alg, const fmm_geom_t& g, bool a_mn, bool b_mn, void* host, const void* dev,
+                          size_t cap, size_t* used, DescView* v)
+{
+    const int R = alg->R;
+    const long nf = alg->nnz_u + alg->nnz_v + alg->nnz_w, np = presum_list_entries(alg);
+    const size_t ptr_bytes = sizeof(int32_t) * 3 * (size_t)(R + 1);
+    const size_t ent_off = (ptr_bytes + 255) & ~(size_t)255;
+    const size_t off2 = (list_bytes(R, nf) + 255) & ~(size_t)255;
+    const size_t need = off2 + list_bytes(R, np);
+    const size_t sizes_off = (need + 255) & ~(size_t)255;    // work space of the builder, not uploaded
+    const size_t total = sizes_off + sizeof(long) * 2 * (size_t)(alg->mt + alg->kt + alg->nt);
+    if (total > cap) return FMMHIP_ERR_INVALID;
+    char* h = (char*)host;
+    const char* d = (const char*)dev;
+    memset(h, 0, need);
+    int32_t* fptr = (int32_t*)h;
+    fmm_entry_t* fent = (fmm_entry_t*)(h + ent_off);
+    fmm_build_entries_into(alg, &g, fptr, fent, (long*)(h + sizes_off));
+    int32_t* pptr = (int32_t*)(h + off2);
+    fmm_entry_t* pent = (fmm_entry_t*)(h + off2 + ent_off);
+    build_presum_list_into(alg, g, fptr, fent, a_mn, b_mn, pptr, pent);
+    memset(v, 0, sizeof(*v));
+    v->fused_ptr = (const int*)d;
+    v->fused_ent = (const fmm_entry_t*)(d + ent_off);
+    v->fused_ent_h = fent;
+    v->presum_ptr = (const int*)(d + off2);
+    v->presum_ent = (const fmm_entry_t*)(d + off2 + ent_off);
+    *used = need;
+    return FMMHIP_SUCCESS;
+}
+
 bool timing_active()
 {
     std::lock_guard<std::mutex> lk(g_tm_mu);
@@ -1279,25 +1369,31 @@ extern "C" int fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m,
         // planes always staged; products materialised as bf16x3_gemm decides
         bool wmat = alg->R > 1 && (alg->pack == 2 || alg->pack < 0);
         if (wmat && alg->pack < 0) {
-            size_t fr = 0, tot = 0;
             const double lm = (double)((m + alg->mt - 1) / alg->mt), ln = (double)((n + alg->nt - 1) / alg->nt);
-            wmat = hipMemGetInfo(&fr, &tot) == hipSuccess && 8.0 * alg->R * lm * ln < 0.5 * (double)fr;
+            wmat = 8.0 * alg->R * lm * ln < fmmhip_ws_bound();
         }
         *pack = wmat ? 2 : 1;
         return FMMHIP_SUCCESS;
     }
     const Kind kind = leaf == FMMHIP_LEAF_F32 ? Kind::D_F32LEAF : Kind::D64;
-    fmm_geom_t g;
-    memset(&g, 0, sizeof(g));
-    g.m = m; g.n = n; g.k = k;
-    g.lm = (m + alg->mt - 1) / alg->mt;
-    g.lk = (k + alg->kt - 1) / alg->kt;
-    g.ln = (n + alg->nt - 1) / alg->nt;
-    g.rsA = 1; g.csA = m; g.rsB = 1; g.csB = k; g.rsC = 1; g.csC = m;
-    const WsLayout wa = ws_layout_a(g, true), wb = ws_layout_b(g, false);
-    const size_t offB = (8 * wa.elems(alg->R) + 255) & ~(size_t)255;
-    const size_t ws_bytes = offB + 8 * wb.elems(alg->R) + 256;
-    *pack = resolve_pack(alg, kind, g, alg->pack, alg->split, 8, ws_bytes, (double)m * (double)n, true);
+    if (m == 0 || n == 0 || k == 0) { *pack = 0; return FMMHIP_SUCCESS; }
+    // the call's own resolution (whole workspace against the bound) on
+    // placeholder, 256-byte aligned operand addresses: nothing is launched
+    const void* ph = (const void*)(uintptr_t)4096;
+    Prepared P;
+    std::unique_lock<std::mutex> lk = enqueue_lock();
+    rc = prepare_call(alg, kind, m, n, k, 1.0, ph, 1, std::max(m, 1L), ph, 1, std::max(k, 1L), 1.0, (void*)ph, 1,
+                      std::max(m, 1L), alg->pack, alg->split, (hipStream_t)fmmhip_default_stream(), &P);
+    if (rc) return rc;
+    *pack = P.mode;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_alg_last_pack(fmmhip_alg_t alg, int* pack)
+{
+    if (!alg || !pack) return FMMHIP_ERR_INVALID;
+    std::unique_lock<std::mutex> lk = enqueue_lock();
+    *pack = alg->last_pack;
     return FMMHIP_SUCCESS;
 }
 
@@ -1409,4 +1505,73 @@ extern "C" int fmmhip_sgemm(long m, long n, long k, float alpha, const float* A,
     if (rc) return rc;
     return gemm_common(classical_alg(), Kind::S32, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C,
                        rsC, csC, stream);
+}
+
+// ---- the pre-summed mode in two stages (the plugin registry's kernels) ----
+//
+// Stage 1 (the pack kernel, plugin/ref_kernels/bli_packm_fmm_ref.c) forms
+// every S_r and T_r of A and B into the caller's workspace; stage 2 (the gemm
+// kernel, plugin/ref_kernels/bli_gemm_fmm_ref.c) runs the R leaf products
+// from it with the W-weighted epilogue into C.  Both resolve the call alike
+// (pre-summed, no product split), so stage 1 then stage 2 on one stream is
+// the one-call pre-summed product bit for bit.
+static int staged_common(int stage, fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha,
+                         const void* A, long rsA, long csA, const void* B, long rsB, long csB, double beta, void* C,
+                         long rsC, long csC, void* ws, size_t ws_bytes, size_t* need, void* stream)
+{
+    int rc = fmmhip_init();
+    if (rc) return rc;
+    if (!alg) return FMMHIP_ERR_INVALID;
+    hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
+    std::unique_lock<std::mutex> lk = enqueue_lock();
+    Prepared P;
+    rc = prepare_call(alg, kind, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, 1, 1, st, &P);
+    if (rc) return rc;
+    if (need) { *need = P.ws_bytes; return FMMHIP_SUCCESS; }
+    if (P.ws_bytes) {
+        if (!ws || ws_bytes < P.ws_bytes) return FMMHIP_ERR_INVALID;
+        bind_workspace(&P, ws);
+    }
+    return launch_prepared(&P, stage);
+}
+
+extern "C" int fmmhip_fmm_workspace_bytes(fmmhip_alg_t alg, int is_double, long m, long n, long k, long rsA,
+                                          long csA, long rsB, long csB, long rsC, long csC, size_t* bytes)
+{
+    if (!bytes) return FMMHIP_ERR_INVALID;
+    const void* ph = (const void*)(uintptr_t)4096;   // placeholder, aligned: nothing is launched
+    return staged_common(0, alg, is_double ? Kind::D64 : Kind::S32, m, n, k, 1.0, ph, rsA, csA, ph, rsB, csB, 1.0,
+                         (void*)ph, rsC, csC, nullptr, 0, bytes, nullptr);
+}
+
+extern "C" int fmmhip_dpack_fmm(fmmhip_alg_t alg, long m, long n, long k, double alpha, const double* A, long rsA,
+                                long csA, const double* B, long rsB, long csB, double beta, double* C, long rsC,
+                                long csC, void* ws, size_t ws_bytes, void* stream)
+{
+    return staged_common(1, alg, Kind::D64, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, ws,
+                         ws_bytes, nullptr, stream);
+}
+
+extern "C" int fmmhip_dgemm_fmm_packed(fmmhip_alg_t alg, long m, long n, long k, double alpha, const double* A,
+                                       long rsA, long csA, const double* B, long rsB, long csB, double beta,
+                                       double* C, long rsC, long csC, void* ws, size_t ws_bytes, void* stream)
+{
+    return staged_common(2, alg, Kind::D64, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, ws,
+                         ws_bytes, nullptr, stream);
+}
+
+extern "C" int fmmhip_spack_fmm(fmmhip_alg_t alg, long m, long n, long k, float alpha, const float* A, long rsA,
+                                long csA, const float* B, long rsB, long csB, float beta, float* C, long rsC,
+                                long csC, void* ws, size_t ws_bytes, void* stream)
+{
+    return staged_common(1, alg, Kind::S32, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, ws,
+                         ws_bytes, nullptr, stream);
+}
+
+extern "C" int fmmhip_sgemm_fmm_packed(fmmhip_alg_t alg, long m, long n, long k, float alpha, const float* A,
+                                       long rsA, long csA, const float* B, long rsB, long csB, float beta, float* C,
+                                       long rsC, long csC, void* ws, size_t ws_bytes, void* stream)
+{
+    return staged_common(2, alg, Kind::S32, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, ws,
+                         ws_bytes, nullptr, stream);
 }

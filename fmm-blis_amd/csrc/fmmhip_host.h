@@ -44,6 +44,8 @@ struct Prepared {
     int shape;
     bool a_mn, b_mn, c_col;
     long nwg;
+    int mode;                    // pack mode taken: 0 fused, 1 pre-summed, 2 products materialised
+    bool untimed;                // never recorded by the launch timing
     bool presum;                 // pre-summed pack mode
     size_t ws_bytes;             // workspace: summation leaves, then the split temporaries
     fmmhip_presum_seg_t segA, segB;
@@ -92,6 +94,12 @@ size_t desc_bound(fmmhip_alg_t alg);
 int desc_pack(fmmhip_alg_t alg, const fmm_geom_t& g, bool presum, bool mat, bool a_mn, bool b_mn, void* host,
               const void* dev, size_t cap, size_t* used, DescView* v);
 
+// The fused and pre-summed lists of one call built directly into `host`
+// (cap bytes; no allocation), in the device layout of desc_pack with
+// pointers bound to `dev`; *used = the bytes to copy to dev.
+int desc_pack_presum_into(fmmhip_alg_t alg, const fmm_geom_t& g, bool a_mn, bool b_mn, void* host, const void* dev,
+                          size_t cap, size_t* used, DescView* v);
+
 // Resolves one call (pack: 0 fused, 1 pre-summed, 2 pre-summed with the
 // products materialised, -1 automatic; split: product groups per tile, 1,
 // 2, 4 or 8, -1 chosen from the tile count; with pack 2 any count up to R).
@@ -100,8 +108,10 @@ int prepare_call(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
                  int split, hipStream_t st, Prepared* P, DescSource* src = nullptr);
 // Rebases a pre-summed call's workspace offsets onto `ws`.
 void bind_workspace(Prepared* P, void* ws);
-// Launches a prepared, bound call on its stream.
-int launch_prepared(const Prepared* P);
+// Launches a prepared, bound call on its stream: stages & 1 the summation
+// stage, stages & 2 the MFMA kernel and what follows it (split reduction, W
+// stage); the launch timing records only whole calls.
+int launch_prepared(const Prepared* P, int stages = 3);
 // Groups per C tile (1, 2, 4, 8; at most max_groups) for a grid of `tiles`
 // workgroups: `requested` when > 0, else chosen by a time model from the
 // leaf flops, the bytes of C, the leaf's MFMA peak (0: FP64 or FP32 by

@@ -359,6 +359,17 @@ extern "C" int fmmhip_mem_info(size_t* f, size_t* t)
     return 0;
 }
 
+extern "C" double fmmhip_ws_bound(void)
+{
+    static std::once_flag once;
+    static double bound = 0.0;
+    std::call_once(once, [] {
+        size_t f = 0, t = 0;
+        if (fmmhip_init() == 0 && hipMemGetInfo(&f, &t) == hipSuccess) bound = 0.5 * (double)t;
+    });
+    return bound;
+}
+
 extern "C" int fmmhip_pointer_is_device(const void* p)
 {
     if (fmmhip_init()) return -1;
@@ -487,6 +498,34 @@ extern "C" int fmmhip_scale_s(float* C, long m, long n, long rs, long cs, float 
     INIT_OR_RETURN();
     if (!m || !n || beta == 1.0f) return 0;
     hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(m * n)), dim3(256), 0, S(s), C, m, n, rs, cs, beta);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// Y := X for a general-strided m x n matrix X (element (i,j) at
+// X[i*rs + j*cs]) into column-major Y (ld = m).  Consecutive threads write
+// consecutive elements of Y; the reads follow X's strides.
+template <typename T>
+__global__ void gather_kernel(const T* __restrict__ X, long m, long n, long rs, long cs, T* __restrict__ Y)
+{
+    const long total = m * n;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long j = idx / m, i = idx - j * m;
+        Y[idx] = X[i * rs + j * cs];
+    }
+}
+
+extern "C" int fmmhip_gather(int is_double, const void* X, long m, long n, long rs, long cs, void* Y, void* s)
+{
+    INIT_OR_RETURN();
+    if (!m || !n) return 0;
+    if (is_double)
+        hipLaunchKernelGGL(gather_kernel<double>, dim3(grid_for(m * n)), dim3(256), 0, S(s), (const double*)X, m, n,
+                           rs, cs, (double*)Y);
+    else
+        hipLaunchKernelGGL(gather_kernel<float>, dim3(grid_for(m * n)), dim3(256), 0, S(s), (const float*)X, m, n,
+                           rs, cs, (float*)Y);
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -105,6 +105,9 @@ _SIGS = {
     "fmm_obj_attach": (None, [ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, _P, ctypes.c_int64,
                               ctypes.c_int64, ctypes.POINTER(ObjT)]),
     "fmm_obj_scalar": (None, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ObjT)]),
+    "fmm_blis_amd_alg_cache_stats": (None, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "bli_plugin_init_fmm_blis_gfx950_ref": (None, []),
+    "fmm_blis_get_ukr_dt": (_P, [ctypes.c_uint32, ctypes.c_uint64]),
     # native
     "fmmhip_status_string": (ctypes.c_char_p, [_I]),
     "fmmhip_init": (_I, []),
@@ -118,6 +121,7 @@ _SIGS = {
     "fmmhip_alg_set_shape": (_I, [_P, _I]),
     "fmmhip_alg_set_pack": (_I, [_P, _I]),
     "fmmhip_alg_pack_for": (_I, [_P, _I, _L, _L, _L, ctypes.POINTER(_I)]),
+    "fmmhip_alg_last_pack": (_I, [_P, ctypes.POINTER(_I)]),
     "fmmhip_alg_set_split": (_I, [_P, _I]),
     "fmmhip_workspace_release": (_I, []),
     "fmmhip_timing_enable": (_I, [_I]),
@@ -129,6 +133,14 @@ _SIGS = {
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm": (_I, [_L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
+    "fmmhip_fmm_workspace_bytes": (_I, [_P, _I, _L, _L, _L, _L, _L, _L, _L, _L, _L,
+                                        ctypes.POINTER(ctypes.c_size_t)]),
+    "fmmhip_dpack_fmm": (_I, [_P, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P, ctypes.c_size_t, _P]),
+    "fmmhip_dgemm_fmm_packed": (_I, [_P, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P,
+                                     ctypes.c_size_t, _P]),
+    "fmmhip_spack_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P, ctypes.c_size_t, _P]),
+    "fmmhip_sgemm_fmm_packed": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P,
+                                     ctypes.c_size_t, _P]),
     "fmmhip_batch_create": (_I, [ctypes.POINTER(_P)]),
     "fmmhip_batch_destroy": (None, [_P]),
     "fmmhip_batch_add_d": (_I, [_P, _L, _L, _L, _D, _P, _L, _P, _L, _D, _P, _L, _I]),
@@ -242,9 +254,15 @@ class Alg:
 
     def pack_for(self, m, n, k, leaf=0):
         """Pack mode (0 fused, 1 pre-summed, 2 products materialised) a
-        column-major FP64 call of m x n x k would use now."""
+        column-major FP64 call of m x n x k resolves to."""
         p = ctypes.c_int(-1)
         check(lib().fmmhip_alg_pack_for(self.h, leaf, m, n, k, ctypes.byref(p)), "alg_pack_for")
+        return p.value
+
+    def last_pack(self):
+        """Pack mode the last FP64/FP32-leaf call took (-1 before the first)."""
+        p = ctypes.c_int(-1)
+        check(lib().fmmhip_alg_last_pack(self.h, ctypes.byref(p)), "alg_last_pack")
         return p.value
 
     def set_split(self, groups):

@@ -102,15 +102,34 @@ typedef struct fmm_s {
 } fmm_t;
 
 /* Kernel slot ids.  Replaces plugin/bli_plugin_fmm_blis.h:45-46; set by
- * bli_plugin_register_fmm_blis (the device kernels are reached through the
- * library, not through a BLIS context). */
+ * bli_plugin_register_fmm_blis, which registers the two slots in order
+ * (ids 1 and 2) and fills them through the per-configuration init below;
+ * fmm_blis_get_ukr_dt reads them back. */
 extern siz_t FMM_BLIS_PACK_UKR;
 extern siz_t FMM_BLIS_GEMM_UKR;
 
-/* Initialises the HIP device and the kernel registry once (thread-safe).
- * Returns BLIS_SUCCESS (-1), or BLIS_FAILURE (-2) when no usable gfx950
- * device is present.  Replaces plugin/bli_plugin_register.c:43-64. */
+/* Initialises the HIP device, registers the two kernel slots and runs the
+ * per-configuration init, once (thread-safe).  Returns BLIS_SUCCESS (-1),
+ * or BLIS_FAILURE (-2) when no usable gfx950 device is present.
+ * Replaces plugin/bli_plugin_register.c:43-64. */
 err_t bli_plugin_register_fmm_blis(void);
+
+/* Per-configuration init of the one configuration, the gfx950 device:
+ * stores the float and double pack kernels (fmmhip_spack_fmm /
+ * fmmhip_dpack_fmm) in slot FMM_BLIS_PACK_UKR and the gemm kernels
+ * (fmmhip_sgemm_fmm_packed / fmmhip_dgemm_fmm_packed) in slot
+ * FMM_BLIS_GEMM_UKR.  Called by bli_plugin_register_fmm_blis.
+ * Replaces plugin/ref_kernels/bli_plugin_init_ref.c:61-71
+ * (bli_plugin_init_fmm_blis_<cfg>_ref). */
+void bli_plugin_init_fmm_blis_gfx950_ref(void);
+
+/* The kernel registered in slot ukr_id for datatype dt (BLIS_FLOAT or
+ * BLIS_DOUBLE), or NULL (unknown slot, complex type, no device); cast it to
+ * fmm_ukr_s_ft / fmm_ukr_d_ft.  Registers first if needed.  The query a
+ * caller of the reference makes with bli_cntx_get_ukr_dt(dt, ukr_id, cntx)
+ * on the context the plugin's init filled (blis/frame/base/bli_cntx.h);
+ * there is one context here, so it takes none. */
+void* fmm_blis_get_ukr_dt(num_t dt, siz_t ukr_id);
 
 /* Built-in <2,2,2;7> Strassen, identical to 222.txt.
  * Replaces frame/base/bli_strassen_ab.c:12-24. */
@@ -170,6 +189,11 @@ void bli_gemm_hip(obj_t* alpha, obj_t* A, obj_t* B, obj_t* beta, obj_t* C);
  * aborting; fmm_blis_amd_last_error returns the last one (0 = none). */
 void fmm_blis_amd_set_abort_on_error(int on);
 int  fmm_blis_amd_last_error(void);
+
+/* The object entries cache up to 16 composed algorithms by table content;
+ * a call pins the one it uses, and eviction takes unpinned ones only.
+ * Reports the cached and the pinned count (diagnostics). */
+void fmm_blis_amd_alg_cache_stats(int* cached, int* pinned);
 
 /* Fills *obj as a view of an existing buffer, like BLIS
  * bli_obj_create_with_attached_buffer (blis/frame/base/bli_obj.c:59-75):
@@ -233,18 +257,27 @@ int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
  * for two-level tables whose W stage runs in two stages, inner level then
  * outer (333, 444: fewer bytes, sums re-associated); the batch uses 1
  * instead); -1 = chosen per call (default: for multi-part
- * algorithms pre-summed when its workspace fits in half of the free
- * device memory, materialised too when that fits as well and a time model
- * expects it to be faster).  No counterpart in the reference, whose packing
+ * algorithms pre-summed when the call's whole workspace fits in half of
+ * the device's total memory, materialised too when that fits as well and a
+ * time model expects it to be faster; the choice depends on the algorithm,
+ * the shape, the layouts and the device, never on what memory is free, so
+ * repeated calls round alike.  Only when the workspace cannot be allocated
+ * does the call fall back to 0 without split, and fmmhip_alg_last_pack
+ * reports it).  No counterpart in the reference, whose packing
  * (plugin/ref_kernels/bli_packm_fmm_ref.c) and scatter
  * (plugin/ref_kernels/bli_gemm_fmm_ref.c) are always fused. */
 int  fmmhip_alg_set_pack(fmmhip_alg_t alg, int pack);
 /* The pack mode (0, 1, 2 as above) a column-major FP64 call of m x n x k
- * with this algorithm and leaf would use now (the automatic choice depends
- * on the free device memory).  BF16x3 leaves always stage their operand
- * planes; they report 2 when the products are materialised (pack 2, or -1
- * with several products and the slots in half of the free memory), else 1. */
+ * with this algorithm and leaf resolves to (the call's own resolution, so
+ * the two agree unless the workspace allocation fails).  BF16x3 leaves
+ * always stage their operand planes; they report 2 when the products are
+ * materialised (pack 2, or -1 with several products and the slots in half
+ * of the device's memory), else 1. */
 int  fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, long n, long k, int* pack);
+/* The pack mode the last FP64/FP32-leaf call with this algorithm took (-1
+ * before the first): differs from fmmhip_alg_pack_for only when an
+ * automatic call fell back to the fused mode for lack of device memory. */
+int  fmmhip_alg_last_pack(fmmhip_alg_t alg, int* pack);
 /* Split: 1 = each 128 x 128 tile of C is one workgroup looping over all R
  * products; 2, 4, 8 = that many workgroups per tile, each taking a
  * contiguous range of products (the classical algorithm, R = 1: a
@@ -266,9 +299,11 @@ int fmmhip_timing_enable(int on);
 int fmmhip_timing_read(long* launches, double* kernel_ms, double* presum_ms, double* max_kernel_ms);
 
 /* C := beta*C + alpha*A*B on device pointers, asynchronous on `stream`
- * (NULL = the library's stream).  Element (i,j) of X is X[i*rsX + j*csX];
- * for A and B one of rs, cs must be 1.  The leaf `leaf` applies to the
- * double entry; the float entry always computes FP32 leaves. */
+ * (NULL = the library's stream).  Element (i,j) of X is X[i*rsX + j*csX]
+ * for any non-negative strides; an A or B with neither stride 1 is first
+ * gathered column-major into a per-stream buffer by one bandwidth pass.
+ * The leaf `leaf` applies to the double entry; the float entry always
+ * computes FP32 leaves. */
 int fmmhip_dgemm_fmm(fmmhip_alg_t alg, fmmhip_leaf_t leaf,
                      long m, long n, long k, double alpha,
                      const double* A, long rsA, long csA,
@@ -287,6 +322,43 @@ int fmmhip_sgemm(long m, long n, long k, float alpha,
                  const float* A, long rsA, long csA,
                  const float* B, long rsB, long csB, float beta,
                  float* C, long rsC, long csC, void* stream);
+
+/* ---- the pre-summed mode in two stages: the registry's kernels ---- */
+
+/* Pack (stage 1): forms every S_r = sum U[.][r] A_part and T_r = sum
+ * V[.][r] B_part of the call into `ws` (HBM-bandwidth pass; alpha, beta and
+ * C are not read).  Gemm (stage 2): the R leaf products S_r*T_r on the
+ * matrix cores from `ws`, each added into every C part with alpha*W (A and
+ * B are not read, only their layouts).  Both take the same arguments, one
+ * unit stride per operand, device pointers, asynchronous on `stream`; stage
+ * 1 then stage 2 on one stream equals fmmhip_dgemm_fmm with pack 1 and
+ * split 1 bit for bit.  ws must hold fmmhip_fmm_workspace_bytes bytes
+ * (16-byte aligned; hipMalloc memory is).  The reference's counterparts are
+ * the pack and gemm micro-kernels the plugin registers:
+ * plugin/ref_kernels/bli_packm_fmm_ref.c:50-148 and
+ * plugin/ref_kernels/bli_gemm_fmm_ref.c:39-121. */
+typedef int (*fmm_ukr_d_ft)(fmmhip_alg_t alg, long m, long n, long k, double alpha,
+                            const double* A, long rsA, long csA, const double* B, long rsB, long csB,
+                            double beta, double* C, long rsC, long csC, void* ws, size_t ws_bytes,
+                            void* stream);
+typedef int (*fmm_ukr_s_ft)(fmmhip_alg_t alg, long m, long n, long k, float alpha,
+                            const float* A, long rsA, long csA, const float* B, long rsB, long csB,
+                            float beta, float* C, long rsC, long csC, void* ws, size_t ws_bytes,
+                            void* stream);
+int fmmhip_fmm_workspace_bytes(fmmhip_alg_t alg, int is_double, long m, long n, long k, long rsA, long csA,
+                               long rsB, long csB, long rsC, long csC, size_t* bytes);
+int fmmhip_dpack_fmm(fmmhip_alg_t alg, long m, long n, long k, double alpha, const double* A, long rsA,
+                     long csA, const double* B, long rsB, long csB, double beta, double* C, long rsC,
+                     long csC, void* ws, size_t ws_bytes, void* stream);
+int fmmhip_dgemm_fmm_packed(fmmhip_alg_t alg, long m, long n, long k, double alpha, const double* A,
+                            long rsA, long csA, const double* B, long rsB, long csB, double beta,
+                            double* C, long rsC, long csC, void* ws, size_t ws_bytes, void* stream);
+int fmmhip_spack_fmm(fmmhip_alg_t alg, long m, long n, long k, float alpha, const float* A, long rsA,
+                     long csA, const float* B, long rsB, long csB, float beta, float* C, long rsC,
+                     long csC, void* ws, size_t ws_bytes, void* stream);
+int fmmhip_sgemm_fmm_packed(fmmhip_alg_t alg, long m, long n, long k, float alpha, const float* A,
+                            long rsA, long csA, const float* B, long rsB, long csB, float beta, float* C,
+                            long rsC, long csC, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- batching form: many independent column-major GEMMs, one graph ---- */
 

@@ -24,6 +24,7 @@ HAS_GPU = os.path.exists("/dev/kfd")
 def header_functions():
     text = open(os.path.join(ROOT, "include", "fmm_blis_amd.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"typedef[^;]*\(\s*\*[^;]*;", "", text)   # function-pointer types
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
     keywords = {"if", "sizeof", "defined"}
     return sorted(n for n in names if n not in keywords and not n.isupper())
