@@ -203,7 +203,7 @@ def permuted_strassen(perm):
     algorithm (the Brent equations hold for any product order)."""
     base = F.strassen()
     U, V, W = base.tables()
-    U, V, W = (np.ascontiguousarray(X[:, perm]).astype(np.int32) for X in (U, V, W))
+    U, V, W = (np.ascontiguousarray(np.asarray(X)[:, perm]).astype(np.int32) for X in (U, V, W))
     f = F.FmmT()
     f.m_tilde, f.n_tilde, f.k_tilde, f.R = 2, 2, 2, 7
     keep = [U, V, W]
