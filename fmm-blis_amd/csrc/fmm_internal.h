@@ -80,6 +80,9 @@ int fmmhip_scale_d(double* C, long m, long n, long rsC, long csC, double beta, v
 int fmmhip_scale_s(float* C, long m, long n, long rsC, long csC, float beta, void* stream);
 /* Pointer kind: 1 device (or managed), 0 host, <0 error. */
 int fmmhip_pointer_is_device(const void* p);
+/* Set-up of the no-allocation entry point's scratch on the current device
+ * (fmmhip_noalloc.hip); called by the library's init. */
+int fmmhip_noalloc_setup(void);
 /* The library's default stream (created on first use). */
 void* fmmhip_default_stream(void);
 /* Largest workspace (bytes) an automatic pack-mode choice may take: half of

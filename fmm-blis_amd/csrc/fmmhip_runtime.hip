@@ -38,6 +38,9 @@ static void do_init()
         g_init_status = FMMHIP_ERR_HIP;
         return;
     }
+    // the no-allocation entry point's scratch on this device (a failure
+    // here is reported by that entry point alone)
+    (void)fmmhip_noalloc_setup();
     g_init_status = FMMHIP_SUCCESS;
 }
 
@@ -63,6 +66,7 @@ extern "C" const char* fmmhip_status_string(int s)
     case FMMHIP_ERR_UNSUPPORTED: return "unsupported";
     case FMMHIP_ERR_NODEVICE: return "no gfx950 device";
     case FMMHIP_ERR_IO: return "coefficient file unreadable or malformed";
+    case FMMHIP_ERR_TOO_LARGE: return "problem exceeds the no-allocation scratch";
     default: return "unknown status";
     }
 }

@@ -28,7 +28,8 @@ TRANS_BIT = 1 << 3
 LEAF_NATIVE, LEAF_F32, LEAF_BF16X3, LEAF_BF16X3_6 = 0, 1, 2, 3
 
 STATUS = {0: "success", 1: "invalid argument", 2: "out of memory", 3: "HIP runtime error",
-          4: "unsupported", 5: "no gfx950 device", 6: "coefficient file unreadable or malformed"}
+          4: "unsupported", 5: "no gfx950 device", 6: "coefficient file unreadable or malformed",
+          7: "problem exceeds the no-allocation scratch"}
 
 
 class FmmT(ctypes.Structure):
@@ -132,6 +133,7 @@ _SIGS = {
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
+    "fmmhip_dgemm_fmm_noalloc": (_I, [_P, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm": (_I, [_L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_fmm_workspace_bytes": (_I, [_P, _I, _L, _L, _L, _L, _L, _L, _L, _L, _L,
                                         ctypes.POINTER(ctypes.c_size_t)]),

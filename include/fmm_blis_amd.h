@@ -214,7 +214,8 @@ typedef enum {
     FMMHIP_ERR_HIP         = 3,  /* a HIP runtime call or a launch failed  */
     FMMHIP_ERR_UNSUPPORTED = 4,  /* valid but not supported (e.g. complex) */
     FMMHIP_ERR_NODEVICE    = 5,  /* no usable gfx950 device                */
-    FMMHIP_ERR_IO          = 6   /* coefficient file unreadable/malformed  */
+    FMMHIP_ERR_IO          = 6,  /* coefficient file unreadable/malformed  */
+    FMMHIP_ERR_TOO_LARGE   = 7   /* beyond the no-allocation scratch        */
 } fmmhip_status_t;
 
 const char* fmmhip_status_string(int status);
@@ -313,6 +314,23 @@ int fmmhip_sgemm_fmm(fmmhip_alg_t alg, long m, long n, long k, float alpha,
                      const float* A, long rsA, long csA,
                      const float* B, long rsB, long csB, float beta,
                      float* C, long rsC, long csC, void* stream);
+/* fmmhip_dgemm_fmm with the FP64 leaf in the pre-summed mode without
+ * product split (the same kernels and the same C as fmmhip_dgemm_fmm with
+ * pack 1 and split 1), calling no hipMalloc, hipFree or other allocating
+ * routine: its lists and S_r / T_r leaves live in a static device scratch of
+ * FMMHIP_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends
+ * (8 MiB of lists, the rest leaves: R*(leaf(A) + leaf(B)) doubles, e.g.
+ * one-level Strassen up to 2048^3).  Larger problems return
+ * FMMHIP_ERR_TOO_LARGE; an A or B with neither stride 1 returns
+ * FMMHIP_ERR_UNSUPPORTED.  Every call on one GPU shares its scratch: calls
+ * on one stream are ordered, calls that overlap on one GPU (different
+ * streams) race.  No counterpart in the reference: for callers that
+ * cannot allocate at call time (HIP graph capture, tight serving loops,
+ * harnesses that forbid device allocation). */
+#define FMMHIP_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int fmmhip_dgemm_fmm_noalloc(fmmhip_alg_t alg, long m, long n, long k, double alpha,
+                             const double* A, long rsA, long csA, const double* B, long rsB, long csB,
+                             double beta, double* C, long rsC, long csC, void* stream);
 /* The project's classical FP64 / FP32 MFMA GEMMs (same kernels, R = 1). */
 int fmmhip_dgemm(long m, long n, long k, double alpha,
                  const double* A, long rsA, long csA,
