@@ -6,28 +6,39 @@
 //
 //  1. split (HBM-bound): for every product r, the U- (V-) weighted sum of
 //     the operand's parts is formed in FP64 (as the pre-summed pack mode
-//     does) and written as three bf16 planes with one power-of-two scale per
-//     (row, 256-wide k chunk):
-//         x = 2^e * (d0 + (d1 + s2) 2^-9),
-//     e chosen so that |x| 2^-e < 256 over the chunk row; d0 and d1 are
-//     integers in [-256, 256] (exact in bf16), s2 = x 2^(9-e) - 512 d0 - d1
-//     (|s2| <= 1/2) rounded to bf16 -- the third 9-bit digit, pre-scaled by
-//     2^-9.  Representation error <= 2^-27 * (chunk row max).
+//     does) and written as bf16 planes with one power-of-two scale per
+//     (row, 256-wide k chunk).  With y = x 2^-e, e chosen so that |y| < 256
+//     over the chunk row, the digits are d0 = rint(y), d1 = rint(512 (y -
+//     d0)), then either
+//       three planes:  P0 = d0, P1 = d1, P2 = bf16(s2) with s2 = 512 (y - d0)
+//                      - d1 (|s2| <= 1/2): y = P0 + 2^-9 (P1 + P2),
+//                      representation error <= 2^-27 of the chunk row max;
+//       four planes:   d2 = rint(512 s2), s3 = 512 s2 - d2, P2 = d2 2^-9
+//                      (exact), P3 = bf16(s3 2^-9): y = P0 + 2^-9 (P1 + P2 +
+//                      P3), representation error <= 2^-36 of the row max.
+//     d0, d1, d2 are integers in [-256, 256], exact in bf16.
 //  2. GEMM: one workgroup per 128 x 128 leaf tile of C, all R products; per
 //     32-wide k step, the plane products on v_mfma_f32_16x16x32_bf16 into
-//     three FP32 accumulators,
-//         G0 = sum d0 d0',
-//         G1 = sum d0 d1' + d1 d0' + d0 s2' + s2 d0',
-//         G2 = sum d1 d1'   (+ d1 s2' + s2 d1' with eight terms).
+//     three FP32 accumulators of weights 1, 2^-9, 2^-18:
+//       three planes, eight terms:
+//         G0 = sum P0 P0',  G1 = sum P0 P1' + P1 P0' + P0 P2' + P2 P0',
+//         G2 = sum P1 P1' + P1 P2' + P2 P1'   (six terms: P1 P1' only);
+//       four planes, ten terms:
+//         G0 = sum P0 P0',
+//         G1 = sum P0 P1' + P1 P0' + P0 P2' + P2 P0' + P0 P3' + P3 P0',
+//         G2 = sum P1 P1' + P1 P2' + P2 P1'.
 //     G0 is a sum of at most 256 integer products of magnitude <= 2^16, so
-//     it is exact in FP32 (the rounding of G1 and G2 weighs 2^-33 and less);
-//     at every chunk end the three are recombined in FP64,
-//         acc += 2^(eA_i + eB_j) * (G0 + 2^-9 (G1 + 2^-9 G2)),
-//     and after the last chunk the W epilogue scatters alpha*w*acc into the
-//     C parts (race-free: the workgroup owns the tile in every part).
-//     Six terms drop the products of weight 2^-27 (d1 s2', s2 d1'); on
-//     uniform[-1,1] inputs that gives about 1.6x the normalised error of
-//     eight terms, whose error is that of the representation.
+//     it is exact in FP32; at every chunk end the three are recombined in
+//     FP64, acc += 2^(eA_i + eB_j) * (G0 + 2^-9 (G1 + 2^-9 G2)), and after
+//     the last chunk the W epilogue scatters alpha*w*acc into the C parts
+//     (race-free: the workgroup owns the tile in every part).
+//     Error (normalised, uniform[-1,1]): three planes are limited by the
+//     representation; four planes by the FP32 rounding of G1, which the bf16
+//     MFMA applies after every 8 k-terms of a product (measured): about 13x
+//     below three planes.  FMM algorithms need that margin, as their operand
+//     sums and W recombination amplify the leaf error (two-level Strassen at
+//     16384^3: 3.4e-11 with four planes, 4.4e-10 with three).  Six terms drop
+//     P1 P2', P2 P1' (about 1.6x the eight-term error).
 //
 // Operand planes in HBM: P[r][g][kb][row][32] (kb = k/32): one k step of a
 // 128-row tile is 8 KB contiguous.  Rows are padded to a multiple of 128
@@ -84,9 +95,11 @@ __device__ __forceinline__ u16 bf16_rne(double d)
 // so that the R blocks of one (row block, chunk) run back to back on one
 // XCD (ids are dealt round-robin to the 8 XCDs): the parts they all read
 // come from that XCD's L2 after the first product, not from HBM R times.
-template <bool ROWC>
+template <bool ROWC, int PLANES>
 __global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
 {
+    static_assert(PLANES == 3 || PLANES == 4, "three or four planes");
+    static_assert(PLANES * SROWS * CHUNK * 2 <= SROWS * (CHUNK + 1) * 8, "the planes fit in the sum's LDS");
     __shared__ __attribute__((aligned(16))) double T[SROWS][CHUNK + 1];
     const int xcd = (int)blockIdx.x & 7, t8 = (int)blockIdx.x >> 3;
     const int r = t8 % a.R;
@@ -158,29 +171,39 @@ __global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
 #pragma unroll
     for (int j = 0; j < CHUNK / 16; j++) {
         const int kk = s + 16 * j;
-        u16 p0, p1, p2;
+        u16 p0, p1, p2, p3 = 0;
         if (bad) {
             p0 = 0x7FC0; p1 = 0; p2 = 0;
         } else {
+            // every step is exact in FP64 (y - rint(y) and the scalings by
+            // 512); only the last plane's bf16 rounding loses bits
             const double y = ldexp(x[j], esh);
             const double d0 = rint(y);
             const double r1 = (y - d0) * 512.0;
             const double d1 = rint(r1);
-            const double s2 = r1 - d1;              // third digit times 2^-9
+            const double s2 = r1 - d1;              // |s2| <= 1/2, in units of d1
             p0 = bf16_exact(d0);
             p1 = bf16_exact(d1);
-            p2 = bf16_rne(s2);
+            if (PLANES == 3) {
+                p2 = bf16_rne(s2);
+            } else {
+                const double r2 = s2 * 512.0;
+                const double d2 = rint(r2);
+                p2 = bf16_exact(d2 * 0x1p-9);       // an integer times 2^-9: exact
+                p3 = bf16_rne((r2 - d2) * 0x1p-9);
+            }
         }
         O[(0 * SROWS + row) * CHUNK + kk] = p0;
         O[(1 * SROWS + row) * CHUNK + kk] = p1;
         O[(2 * SROWS + row) * CHUNK + kk] = p2;
+        if (PLANES == 4) O[(3 * SROWS + row) * CHUNK + kk] = p3;
     }
     if (s == 0) a.E[((long)r * (a.ldk / CHUNK) + ch) * a.Lp + row0 + row] = bad ? 0 : -esh;
     __syncthreads();
     // 3. write out: per (plane, k step) SROWS rows of 64 B, contiguous
     const long plane = (long)a.ldk * a.Lp;
 #pragma unroll
-    for (int i = 0; i < 3 * SROWS * CHUNK / 8 / 256; i++) {
+    for (int i = 0; i < PLANES * SROWS * CHUNK / 8 / 256; i++) {
         const int idx = t + 256 * i;
         const int g = idx / (SROWS * CHUNK / 8);
         const int rem = idx % (SROWS * CHUNK / 8);
@@ -188,7 +211,8 @@ __global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
         const int rem2 = rem % (SROWS * 4);
         const int rr = rem2 >> 2, c = rem2 & 3;
         const uint4 v = *(const uint4*)&O[(g * SROWS + rr) * CHUNK + kbl * KSTEP + c * 8];
-        u16* dst = a.P + ((long)r * 3 + g) * plane + ((long)(k0 / KSTEP + kbl) * a.Lp + row0 + rr) * KSTEP + c * 8;
+        u16* dst =
+            a.P + ((long)r * PLANES + g) * plane + ((long)(k0 / KSTEP + kbl) * a.Lp + row0 + rr) * KSTEP + c * 8;
         *(uint4*)dst = v;
     }
 }
@@ -220,11 +244,11 @@ struct GemmArgs {
     long tmp_elems;
 };
 
-template <int WM, int WN, int FI, int FJ>
+template <int WM, int WN, int FI, int FJ, int PLANES>
 struct GCfg {
     static constexpr int NT = WM * WN * 64;
     static constexpr int BM = WM * FI * 16, BN = WN * FJ * 16;
-    static constexpr int A_CH = 3 * BM * 4, B_CH = 3 * BN * 4;   // 16-byte chunks per stage
+    static constexpr int A_CH = PLANES * BM * 4, B_CH = PLANES * BN * 4;   // 16-byte chunks per stage
     static constexpr int A_PER = A_CH / NT, B_PER = B_CH / NT;
     static_assert((BM * 4) % NT == 0 && (BN * 4) % NT == 0, "a plane's chunks divide among threads");
     static_assert(BM + BN <= NT, "one exponent per thread");
@@ -234,10 +258,11 @@ struct GCfg {
 // the 16-byte reads of eight consecutive rows hit distinct banks
 __device__ __forceinline__ int lds_chunk(int row, int c) { return row * 4 + (c ^ ((row >> 1) & 3)); }
 
-template <int WM, int WN, int FI, int FJ, int TERMS>
+template <int WM, int WN, int FI, int FJ, int PLANES, int TERMS>
 __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
 {
-    typedef GCfg<WM, WN, FI, FJ> G;
+    static_assert((PLANES == 3 && (TERMS == 6 || TERMS == 8)) || (PLANES == 4 && TERMS == 10), "plane products");
+    typedef GCfg<WM, WN, FI, FJ, PLANES> G;
     constexpr int BM = G::BM, BN = G::BN, NT = G::NT;
     __shared__ u32x4 sA[2][G::A_CH];
     __shared__ u32x4 sB[2][G::B_CH];
@@ -270,8 +295,8 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
 #define BF3_GLOAD(R_, KB_)                                                                                   \
     do {                                                                                                     \
         const int r_ = (R_), kb_ = (KB_);                                                                    \
-        const u16* Ab_ = a.A + (long)r_ * 3 * a.a_plane + ((long)kb_ * a.LpA + m0) * KSTEP + tid * 8;        \
-        const u16* Bb_ = a.B + (long)r_ * 3 * a.b_plane + ((long)kb_ * a.LpB + n0) * KSTEP + tid * 8;        \
+        const u16* Ab_ = a.A + (long)r_ * PLANES * a.a_plane + ((long)kb_ * a.LpA + m0) * KSTEP + tid * 8;   \
+        const u16* Bb_ = a.B + (long)r_ * PLANES * a.b_plane + ((long)kb_ * a.LpB + n0) * KSTEP + tid * 8;   \
         _Pragma("unroll") for (int i_ = 0; i_ < G::A_PER; i_++)                                              \
             ra[i_] = *(const u32x4*)(Ab_ + (i_ / PA) * a.a_plane + (i_ % PA) * NT * 8);                       \
         _Pragma("unroll") for (int i_ = 0; i_ < G::B_PER; i_++)                                              \
@@ -324,8 +349,75 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
             const u32x4* lb = sB[st];
             auto fa = [&](int g, int i) { return __builtin_bit_cast(bf16x8, la[rd_a + (g * BM + 16 * i) * 4]); };
             auto fb = [&](int g, int j) { return __builtin_bit_cast(bf16x8, lb[rd_b + (g * BN + 16 * j) * 4]); };
-            {
 #define BF3_MFMA(ACC, X, Y) ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, Y, ACC, 0, 0, 0)
+            if (PLANES == 4) {
+                // ten terms, ordered so that few fragments are live at once:
+                // P0 with every B plane, then P1, P2, P3 with theirs
+                bf16x8 a0[FI], b0[FJ], b1[FJ], b2[FJ];
+#pragma unroll
+                for (int i = 0; i < FI; i++) a0[i] = fa(0, i);
+#pragma unroll
+                for (int j = 0; j < FJ; j++) b0[j] = fb(0, j);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) BF3_MFMA(G0[i][j], a0[i], b0[j]);
+#pragma unroll
+                for (int j = 0; j < FJ; j++) b1[j] = fb(1, j);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) BF3_MFMA(G1[i][j], a0[i], b1[j]);
+#pragma unroll
+                for (int j = 0; j < FJ; j++) b2[j] = fb(2, j);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) BF3_MFMA(G1[i][j], a0[i], b2[j]);
+                {
+                    bf16x8 b3[FJ];
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) b3[j] = fb(3, j);
+#pragma unroll
+                    for (int i = 0; i < FI; i++)
+#pragma unroll
+                        for (int j = 0; j < FJ; j++) BF3_MFMA(G1[i][j], a0[i], b3[j]);
+                }
+                {
+                    bf16x8 a1[FI];
+#pragma unroll
+                    for (int i = 0; i < FI; i++) a1[i] = fa(1, i);
+#pragma unroll
+                    for (int i = 0; i < FI; i++)
+#pragma unroll
+                        for (int j = 0; j < FJ; j++) {
+                            BF3_MFMA(G1[i][j], a1[i], b0[j]);
+                            BF3_MFMA(G2[i][j], a1[i], b1[j]);
+                            BF3_MFMA(G2[i][j], a1[i], b2[j]);
+                        }
+                }
+                {
+                    bf16x8 a2[FI];
+#pragma unroll
+                    for (int i = 0; i < FI; i++) a2[i] = fa(2, i);
+#pragma unroll
+                    for (int i = 0; i < FI; i++)
+#pragma unroll
+                        for (int j = 0; j < FJ; j++) {
+                            BF3_MFMA(G1[i][j], a2[i], b0[j]);
+                            BF3_MFMA(G2[i][j], a2[i], b1[j]);
+                        }
+                }
+                {
+                    bf16x8 a3[FI];
+#pragma unroll
+                    for (int i = 0; i < FI; i++) a3[i] = fa(3, i);
+#pragma unroll
+                    for (int i = 0; i < FI; i++)
+#pragma unroll
+                        for (int j = 0; j < FJ; j++) BF3_MFMA(G1[i][j], a3[i], b0[j]);
+                }
+            } else {
                 bf16x8 a0[FI], b0[FJ], a1[FI], b1[FJ];
 #pragma unroll
                 for (int i = 0; i < FI; i++) a0[i] = fa(0, i);
@@ -370,8 +462,8 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
                         BF3_MFMA(G1[i][j], a2[i], b0[j]);
                         if (TERMS == 8) BF3_MFMA(G2[i][j], a2[i], b1[j]);
                     }
-#undef BF3_MFMA
             }
+#undef BF3_MFMA
 
             if (kb % STEPS_PER_CHUNK == STEPS_PER_CHUNK - 1) {
                 // chunk end: FP64 recombination with the chunk's row/column scales
@@ -443,7 +535,7 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
 namespace {
 
 constexpr int kWM = 2, kWN = 4, kFI = 4, kFJ = 2;
-typedef bf3::GCfg<kWM, kWN, kFI, kFJ> Cfg3;
+typedef bf3::GCfg<kWM, kWN, kFI, kFJ, 3> Cfg3;   // tile geometry (independent of the plane count)
 
 long round_up(long x, long m) { return (x + m - 1) / m * m; }
 
@@ -452,7 +544,8 @@ long round_up(long x, long m) { return (x + m - 1) / m * m; }
 int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alpha, const double* A, long rsA, long csA,
                 const double* B, long rsB, long csB, double beta, double* C, long rsC, long csC, hipStream_t st)
 {
-    if (!alg || m < 0 || n < 0 || k < 0 || (terms != 6 && terms != 8)) return FMMHIP_ERR_INVALID;
+    if (!alg || m < 0 || n < 0 || k < 0 || (terms != 6 && terms != 8 && terms != 10)) return FMMHIP_ERR_INVALID;
+    const int planes = terms == 10 ? 4 : 3;
     if (m == 0 || n == 0) return FMMHIP_SUCCESS;
     if (!C) return FMMHIP_ERR_INVALID;
     if (k == 0 || alpha == 0.0) return fmmhip_scale_d(C, m, n, rsC, csC, beta, st);
@@ -479,7 +572,7 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     const long ldk = round_up(g.lk, bf3::CHUNK);
     const long nch = ldk / bf3::CHUNK;
     if (LpA > INT32_MAX || LpB > INT32_MAX || ldk > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
-    const size_t planesA = (size_t)R * 3 * LpA * ldk * 2, planesB = (size_t)R * 3 * LpB * ldk * 2;
+    const size_t planesA = (size_t)R * planes * LpA * ldk * 2, planesB = (size_t)R * planes * LpB * ldk * 2;
     const size_t expA = (size_t)R * nch * LpA * 4, expB = (size_t)R * nch * LpB * 4;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t oPB = al(planesA), oEA = oPB + al(planesB), oEB = oEA + al(expA);
@@ -528,10 +621,17 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     // grid: (row block, chunk) pairs rounded up to 8 (one run per XCD) x products
     const dim3 gA((unsigned)(round_up(sa.nrc, 8) * R));
     const dim3 gB((unsigned)(round_up(sb.nrc, 8) * R));
-    if (rsA == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gA, blk, 0, st, sa);
-    else hipLaunchKernelGGL(bf3::split_kernel<false>, gA, blk, 0, st, sa);
-    if (csB == 1) hipLaunchKernelGGL(bf3::split_kernel<true>, gB, blk, 0, st, sb);
-    else hipLaunchKernelGGL(bf3::split_kernel<false>, gB, blk, 0, st, sb);
+    auto split = [&](const bf3::SplitArgs& sg, bool rowc, dim3 grid) {
+        if (planes == 4) {
+            if (rowc) hipLaunchKernelGGL((bf3::split_kernel<true, 4>), grid, blk, 0, st, sg);
+            else hipLaunchKernelGGL((bf3::split_kernel<false, 4>), grid, blk, 0, st, sg);
+        } else {
+            if (rowc) hipLaunchKernelGGL((bf3::split_kernel<true, 3>), grid, blk, 0, st, sg);
+            else hipLaunchKernelGGL((bf3::split_kernel<false, 3>), grid, blk, 0, st, sg);
+        }
+    };
+    split(sa, rsA == 1, gA);
+    split(sb, csB == 1, gB);
     if (hipGetLastError() != hipSuccess) return FMMHIP_ERR_HIP;
     if (!wmat && groups > 1 && (rc = fmmhip_split_zero(1, ws + oT, tmp_elems, groups, st))) return rc;
     if (timed) timing_record(&e1, st);
@@ -573,12 +673,15 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
     }
     const long nwg = (long)ga.tiles_m * ga.tiles_n * groups;
     if (nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
-    if (terms == 8)
-        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 8>), dim3((unsigned)nwg), dim3(Cfg3::NT), 0,
-                           st, ga);
+    if (terms == 10)
+        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 4, 10>), dim3((unsigned)nwg), dim3(Cfg3::NT),
+                           0, st, ga);
+    else if (terms == 8)
+        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 3, 8>), dim3((unsigned)nwg), dim3(Cfg3::NT),
+                           0, st, ga);
     else
-        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 6>), dim3((unsigned)nwg), dim3(Cfg3::NT), 0,
-                           st, ga);
+        hipLaunchKernelGGL((bf3::bf16x3_gemm_kernel<kWM, kWN, kFI, kFJ, 3, 6>), dim3((unsigned)nwg), dim3(Cfg3::NT),
+                           0, st, ga);
     if (hipGetLastError() != hipSuccess) return FMMHIP_ERR_HIP;
     if (timed) {
         timing_record(&e2, st);

@@ -1365,7 +1365,7 @@ extern "C" int fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m,
     if (!alg || !pack || m < 0 || n < 0 || k < 0) return FMMHIP_ERR_INVALID;
     int rc = fmmhip_init();
     if (rc) return rc;
-    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6) {
+    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6 || leaf == FMMHIP_LEAF_BF16X3_8) {
         // planes always staged; products materialised as bf16x3_gemm decides
         bool wmat = alg->R > 1 && (alg->pack == 2 || alg->pack < 0);
         if (wmat && alg->pack < 0) {
@@ -1468,12 +1468,14 @@ extern "C" int fmmhip_dgemm_fmm(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, lo
     case FMMHIP_LEAF_NATIVE: kind = Kind::D64; break;
     case FMMHIP_LEAF_F32: kind = Kind::D_F32LEAF; break;
     case FMMHIP_LEAF_BF16X3:
-    case FMMHIP_LEAF_BF16X3_6: {
+    case FMMHIP_LEAF_BF16X3_6:
+    case FMMHIP_LEAF_BF16X3_8: {
         int rc = fmmhip_init();
         if (rc) return rc;
         hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
-        return bf16x3_gemm(alg, leaf == FMMHIP_LEAF_BF16X3 ? 8 : 6, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta,
-                           C, rsC, csC, st);
+        // plane products: ten (four planes), six or eight (three planes)
+        const int terms = leaf == FMMHIP_LEAF_BF16X3 ? 10 : leaf == FMMHIP_LEAF_BF16X3_6 ? 6 : 8;
+        return bf16x3_gemm(alg, terms, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, st);
     }
     default: return FMMHIP_ERR_UNSUPPORTED;
     }

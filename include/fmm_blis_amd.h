@@ -170,10 +170,14 @@ typedef enum {
     FMMHIP_LEAF_NATIVE = 0,   /* the operands' type: FP64 for double, FP32 for float */
     FMMHIP_LEAF_F32    = 1,   /* FP64 operands, FP32 MFMA leaf, FP64 writeback       */
     FMMHIP_LEAF_BF16X3 = 2,   /* FP64 operands, split BF16 MFMA leaf (FP32 accumulate),
-                                 FP64 recombination and writeback: three bf16 planes
-                                 per operand, eight plane products                    */
-    FMMHIP_LEAF_BF16X3_6 = 3  /* as BF16X3 with six plane products (faster, about
-                                 1.6x the error)                                      */
+                                 FP64 recombination and writeback: four bf16 planes
+                                 per operand, ten plane products (normalised error
+                                 about 1e-11 for the classical algorithm, a few 1e-11
+                                 for one- and two-level Strassen at 8192^3-16384^3)   */
+    FMMHIP_LEAF_BF16X3_6 = 3, /* three planes, six plane products (fastest; about 1e-10
+                                 classical, 1e-9 two-level Strassen at 16384^3)       */
+    FMMHIP_LEAF_BF16X3_8 = 4  /* three planes, eight plane products (about 6e-11
+                                 classical, 4e-10 two-level Strassen at 16384^3)      */
 } fmmhip_leaf_t;
 
 /* Like bli_strassen_ab_ex with `nlevels` levels; level l uses algs[l]

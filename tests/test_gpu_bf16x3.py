@@ -1,22 +1,31 @@
-"""GPU parity of the BF16x3 leaf (FMMHIP_LEAF_BF16X3, eight plane products,
-and FMMHIP_LEAF_BF16X3_6, six): FP64 operands split into three bf16 planes
-with per-(row, 256-k chunk) power-of-two scales, plane products on the bf16
-MFMA with FP32 accumulation, FP64 recombination and FP64 W write-back.
+"""GPU parity of the split BF16 leaves: FP64 operands split into three or
+four bf16 planes with per-(row, 256-k chunk) power-of-two scales, plane
+products on the bf16 MFMA with FP32 accumulation, FP64 recombination and
+FP64 W write-back.  Leaves: FMMHIP_LEAF_BF16X3 (four planes, ten plane
+products), FMMHIP_LEAF_BF16X3_8 (three planes, eight) and
+FMMHIP_LEAF_BF16X3_6 (three planes, six).
 
-Bars (stated per test; the product split is covered by the last two tests):
+Bars (stated per test; the product split is covered by the split tests):
   * small-integer inputs are split exactly and every chunk sum is an exact
     FP32 integer, so the result equals the oracle's bit for bit, for the
     classical leaf and for 1- and 2-level FMM, on ragged shapes;
   * uniform[-1,1] inputs at 1024^3: normalised error
     ||C - C_ref||_F/(||A||_F ||B||_F) against the oracle's FP64 classical
     product below the bounds in TOL (about 1.5x the error of a float64
-    restatement of the split scheme on the same inputs: 8 terms 2.5e-10
-    classical, 6.6e-10 one-level and 1.7e-9 two-level Strassen; 6 terms
-    4.1e-10, 1.3e-9, 4.3e-9; the error falls as 1/sqrt(k));
+    restatement of the split scheme on the same inputs, with the MFMA's
+    FP32 rounding after every 8 k-terms of a product as measured on the
+    device: 10 terms 1.9e-11 classical, 5.1e-11 one-level and 1.3e-10
+    two-level Strassen; 8 terms 2.5e-10, 6.6e-10, 1.7e-9; 6 terms 4.1e-10,
+    1.3e-9, 4.3e-9; the error falls as 1/sqrt(k));
   * the same with rows of A and columns of B scaled by 10^U(-4,4) (768^3;
-    restatement 8 terms 4.6e-10 classical, 2.1e-9 two-level; 6 terms
-    1.0e-9, 5.3e-9), and elementwise for the classical leaf relative to the
-    row/column scales (restatement 2.5e-8 / 5.9e-8);
+    restatement 10 terms 3.6e-11 classical, 1.4e-10 two-level; 8 terms
+    4.6e-10, 2.1e-9; 6 terms 1.0e-9, 5.3e-9), and elementwise for the
+    classical leaf relative to the row/column scales (restatement 2.4e-9 /
+    2.5e-8 / 5.9e-8);
+  * the precision target at full size: the classical algorithm and one-
+    and two-level Strassen with the FMMHIP_LEAF_BF16X3 leaf at 8192^3 and
+    16384^3, uniform and stress inputs, within 1e-10 of the FP64 classical
+    kernel (which the parity tests pin against the oracle);
   * a NaN in A makes exactly its row of C NaN.
 """
 import sys
@@ -30,13 +39,16 @@ from conftest import alg_path
 
 pytestmark = pytest.mark.gpu
 
-LEAVES = {8: F.LEAF_BF16X3, 6: F.LEAF_BF16X3_6}
+LEAVES = {10: F.LEAF_BF16X3, 8: F.LEAF_BF16X3_8, 6: F.LEAF_BF16X3_6}
 TOL = {  # (terms, algorithm) -> bound
+    (10, "classical"): 3e-11, (10, "222x1"): 7.5e-11, (10, "222x2"): 2e-10,
     (8, "classical"): 4e-10, (8, "222x1"): 1e-9, (8, "222x2"): 2.6e-9,
     (6, "classical"): 6.5e-10, (6, "222x1"): 2e-9, (6, "222x2"): 6.5e-9,
 }
-TOL_STRESS = {(8, "classical"): 7e-10, (8, "222x2"): 3.2e-9, (6, "classical"): 1.6e-9, (6, "222x2"): 8e-9}
-TOL_ELEM = {8: 4e-8, 6: 9e-8}
+TOL_STRESS = {(10, "classical"): 5.5e-11, (10, "222x2"): 2.2e-10, (8, "classical"): 7e-10, (8, "222x2"): 3.2e-9,
+              (6, "classical"): 1.6e-9, (6, "222x2"): 8e-9}
+TOL_ELEM = {10: 4e-9, 8: 4e-8, 6: 9e-8}
+TARGET = 1e-10     # the precision form's bound against the FP64 product
 
 
 def say(*a):
@@ -59,6 +71,7 @@ def dev(X):
 
 
 def run(alg, alpha, A, B, beta, C, terms=8):
+    """terms: 10, 8 or 6 (the leaf with that many plane products)."""
     m, k = A.shape
     n = B.shape[1]
     dA, dB, dC = dev(A), dev(B), dev(C)
@@ -78,7 +91,7 @@ def algs():
 SHAPES = [(1, 1, 1), (3, 13, 7), (96, 96, 1), (129, 130, 257), (300, 280, 600), (64, 512, 300)]
 
 
-@pytest.mark.parametrize("terms", [8, 6])
+@pytest.mark.parametrize("terms", [10, 8, 6])
 @pytest.mark.parametrize("name", ["classical", "222x1", "222x2", "333x1"])
 def test_bf16x3_exact_integers(name, terms):
     alg, og = algs()[name]
@@ -92,7 +105,7 @@ def test_bf16x3_exact_integers(name, terms):
             assert np.array_equal(got, want), (m, n, k, alpha, beta)
 
 
-@pytest.mark.parametrize("terms", [8, 6])
+@pytest.mark.parametrize("terms", [10, 8, 6])
 @pytest.mark.parametrize("name", ["classical", "222x1", "222x2"])
 def test_bf16x3_random_error(name, terms):
     alg, _ = algs()[name]
@@ -105,7 +118,7 @@ def test_bf16x3_random_error(name, terms):
     assert err < TOL[(terms, name)]
 
 
-@pytest.mark.parametrize("terms", [8, 6])
+@pytest.mark.parametrize("terms", [10, 8, 6])
 @pytest.mark.parametrize("name", ["classical", "222x2"])
 def test_bf16x3_scaled_rows_and_columns(name, terms):
     """Stress variant: rows of A and columns of B scaled by 10^U(-4,4)."""
@@ -128,6 +141,43 @@ def test_bf16x3_scaled_rows_and_columns(name, terms):
         rel = np.abs(got - ref) / scale
         say(f"  max scaled elementwise error {rel.max():.3e}")
         assert rel.max() < TOL_ELEM[terms]
+
+
+def stress_scales(n, seed):
+    rng = np.random.default_rng(seed)
+    return 10.0 ** rng.uniform(-4, 4, size=n)
+
+
+@pytest.mark.parametrize("n", [8192, 16384])
+def test_bf16x3_precision_target_at_full_size(n):
+    """The classical algorithm and one- and two-level Strassen with the
+    FMMHIP_LEAF_BF16X3 leaf at n^3 (the precision form's sizes),
+    uniform[-1,1] inputs and the stress variant (rows of A, columns of B
+    scaled by 10^U(-4,4)): normalised error against the FP64 classical
+    kernel on the same operands <= 1e-10."""
+    A = F.DeviceMatrix(n, n).fill_uniform(0, 0)
+    B = F.DeviceMatrix(n, n).fill_uniform(0, 1)
+    ref, C = F.DeviceMatrix(n, n), F.DeviceMatrix(n, n)
+    s = F.strassen()
+    for inputs in ("uniform", "stress"):
+        if inputs == "stress":
+            rs, cs = F.DeviceBuffer(8 * n), F.DeviceBuffer(8 * n)
+            rs.upload(stress_scales(n, 1))
+            cs.upload(stress_scales(n, 2))
+            F.check(F.lib().fmmhip_scale_rows_cols(A.ptr, n, n, n, rs.ptr, None, None), "scale A rows")
+            F.check(F.lib().fmmhip_scale_rows_cols(B.ptr, n, n, n, None, cs.ptr, None), "scale B cols")
+        F.dgemm(n, n, n, 1.0, A, B, 0.0, ref)
+        F.sync()
+        nA, nB = F.fro(A), F.fro(B)
+        for levels in (0, 1, 2):
+            say(f"bf16x3 precision target: {inputs} 222 L{levels} at {n}^3")
+            alg = F.Alg(classical=True) if levels == 0 else F.Alg([s] * levels)
+            F.dgemm_fmm(alg, n, n, n, 1.0, A, B, 0.0, C, leaf=F.LEAF_BF16X3)
+            F.sync()
+            err = F.fro_diff(C, ref) / (nA * nB)
+            say(f"  normalised error {err:.3e} (bound {TARGET:g})")
+            assert err <= TARGET
+    F.check(F.lib().fmmhip_workspace_release(), "workspace_release")
 
 
 def test_bf16x3_nan_row():

@@ -158,7 +158,7 @@ def test_pack_for_reports_the_mode():
 
 
 @pytest.mark.parametrize("name,levels", [("222", 1), ("222", 2), ("333", 1)])
-@pytest.mark.parametrize("leaf", [F.LEAF_BF16X3, F.LEAF_BF16X3_6])
+@pytest.mark.parametrize("leaf", [F.LEAF_BF16X3, F.LEAF_BF16X3_8, F.LEAF_BF16X3_6])
 def test_bf16x3_materialised_equals_in_kernel_epilogue(name, levels, leaf):
     """BF16x3 leaf: with the products materialised (pack 2, and -1 by
     default) the kernel only stores each product and the W stage applies W;
