@@ -130,6 +130,7 @@ _SIGS = {
     "fmmhip_memcpy_d2d": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "fmmhip_copy_kernel": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "fmmhip_measure_peaks": (_I, [ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "fmmhip_measure_hbm": (_I, [ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "fmmhip_dgemm_fmm": (_I, [_P, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
     "fmmhip_sgemm_fmm": (_I, [_P, _L, _L, _L, _F, _P, _L, _L, _P, _L, _L, _F, _P, _L, _L, _P]),
     "fmmhip_dgemm": (_I, [_L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P]),
@@ -555,6 +556,13 @@ def measure_peaks():
     t, g = _D(), _D()
     check(lib().fmmhip_measure_peaks(ctypes.byref(t), ctypes.byref(g)), "measure_peaks")
     return t.value, g.value
+
+
+def measure_hbm():
+    """(copy, read-only, write-only) GB/s: the device's streaming ceilings."""
+    c, r, w = _D(), _D(), _D()
+    check(lib().fmmhip_measure_hbm(ctypes.byref(c), ctypes.byref(r), ctypes.byref(w)), "measure_hbm")
+    return c.value, r.value, w.value
 
 
 def timing_enable(on=True):

@@ -481,9 +481,15 @@ int fmmhip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int fmmhip_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 /* Measured peaks (no counterpart in the reference; for rooflines): the FP64
  * MFMA rate of back-to-back independent v_mfma_f64_16x16x4_f64 chains on
- * every SIMD (TFLOP/s), and the read+write rate of a 4 GiB copy (GB/s).
- * Takes about a second and 8 GiB of device memory. */
+ * every SIMD (TFLOP/s), and the read+write rate of a 4 GiB copy (GB/s, the
+ * faster of the plain and the streaming copy of fmmhip_measure_hbm).  Takes
+ * about two seconds and 8 GiB of device memory. */
 int fmmhip_measure_peaks(double* fp64_mfma_tflops, double* copy_gbs);
+/* The device's practical HBM ceilings (GB/s, best of grids of 1, 2 and 4
+ * blocks per CU over 4 GiB buffers, 16-byte non-temporal accesses): a copy
+ * with four loads in flight per thread (read + write bytes), a read-only and
+ * a write-only stream.  Takes under a second and 8 GiB of device memory. */
+int fmmhip_measure_hbm(double* copy_gbs, double* read_gbs, double* write_gbs);
 int fmmhip_stream_create(void** stream);
 int fmmhip_stream_destroy(void* stream);
 int fmmhip_stream_sync(void* stream);
