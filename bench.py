@@ -121,10 +121,12 @@ def run_batch(args, F):
     flops = sum(2.0 * m * n * k for (_, m, n, k) in shapes)
     flops64 = sum(2.0 * m * n * k for (dt, m, n, k) in shapes if dt == "f8")
 
-    def make(cand):
+    def make(cand, tile=0):
         b = F.Batch()
-        for (dt, m, n, k), (A, B, C) in zip(shapes, mats):
+        for i, ((dt, m, n, k), (A, B, C)) in enumerate(zip(shapes, mats)):
             b.add(m, n, k, 1.0, A, B, 1.0, C, cand=cand)
+            if tile:
+                b.set_tile(i, tile)
         t0 = time.perf_counter()
         b.plan()
         return b, time.perf_counter() - t0
@@ -146,10 +148,12 @@ def run_batch(args, F):
     tuned, plan_s = make(-1)
     info = tuned.info()
     choices = {"f64": [0, 0, 0], "f32": [0, 0, 0]}
+    tiles = {"f64": {"128": 0, "64": 0}, "f32": {"128": 0, "64": 0}}
     leaf64 = leaf32 = 0.0
     for i, (dt, m, n, k) in enumerate(shapes):
         c = tuned.choice(i)
         choices["f64" if dt == "f8" else "f32"][c] += 1
+        tiles["f64" if dt == "f8" else "f32"][str(tuned.tile(i))] += 1
         d = 2 ** c
         fl = 2.0 * (7 ** c) * -(-m // d) * -(-n // d) * -(-k // d)
         if dt == "f8":
@@ -161,6 +165,13 @@ def run_batch(args, F):
     cls, _ = make(0)
     cls_ms, _ = timed(cls.run, max(1, args.classical_steps), 1) if args.classical_steps > 0 else (None, None)
     cls.close()
+    # the same choices with every problem on 128 x 128 tiles: what the
+    # per-problem tile choice gains
+    t128_ms = None
+    if args.classical_steps > 0:
+        t128, _ = make(-1, 128)
+        t128_ms, _ = timed(t128.run, max(1, args.classical_steps), 1)
+        t128.close()
 
     seq_ms = None
     if args.classical_steps > 0:
@@ -191,12 +202,14 @@ def run_batch(args, F):
                          f"first half FP64, rest FP32, one captured HIP graph"),
             "problems": len(shapes), "total_flops": flops, "fp64_flops": flops64,
             "choices_classical_1level_2level": choices,
+            "tile_edges": tiles,
             "items_f64": info["items_f64"], "items_f32": info["items_f32"],
             "workspace_bytes": info["workspace_bytes"], "tuner_est_ms": round(info["est_ms"], 3),
             "plan_s": round(plan_s, 3),
             "classical_batch_ms": round(cls_ms, 3) if cls_ms else None,
             "classical_batch_gflops": round(flops / (cls_ms * 1e-3) / 1e9, 1) if cls_ms else None,
             "tuned_over_classical": round(cls_ms / ms, 4) if cls_ms else None,
+            "all_128_tiles_batch_ms": round(t128_ms, 3) if t128_ms else None,
             "sequential_classical_calls_ms": round(seq_ms, 3) if seq_ms else None,
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},

@@ -25,21 +25,28 @@ namespace {
 // ---- tuner -------------------------------------------------------------------
 
 // Candidates: 0 classical, 1 one-level Strassen, 2 two-level Strassen (both
-// pre-summed).  Cost model of one candidate on one problem:
-//   t = F_leaf / (P * eff * nk/(nk + ovh)) * quant + bytes_sum / bw
-// F_leaf the executed leaf flops, P the MFMA peak of the datatype, eff the
-// kernel's sustained fraction of peak, nk the 16-wide k steps per product,
-// ovh the per-product prologue/epilogue cost in k steps, quant the
-// wave-quantisation loss of a lone launch without product split (1 inside
-// a batch), bytes_sum the
-// summation stage's traffic and bw its bandwidth.
+// pre-summed), each on one of two tiles: 128 x 128 (launch shape 0) or
+// 64 x 64 (launch shape 2).  Cost model of one candidate and tile on one
+// problem:
+//   t = F_leaf * pad / (P * eff * teff * nk/(nk + ovh)) * quant + bytes_sum / bw
+// F_leaf the leaf flops, pad the tile grid's padding (tiles * T^2 over the
+// leaf's lm * ln: the kernel runs whole tiles), P the MFMA peak of the
+// datatype, eff the 128-tile kernel's sustained fraction of peak, teff the
+// tile's rate relative to it on whole tiles, nk the 16-wide k steps per
+// product, ovh the per-product prologue/epilogue cost in k steps, quant the
+// wave-quantisation loss of a lone launch without product split (1 inside a
+// batch), bytes_sum the summation stage's traffic and bw its bandwidth.
 constexpr int kCands = 3;
+constexpr int kTiles = 2;                         // 0: 128 x 128, 1: 64 x 64
+const int kTileShape[kTiles] = {0, 2};            // their kernel launch shapes
+const int kTileWgPerCu[kTiles] = {2, 4};          // resident workgroups per CU
 struct TunerParams {
     double eff[2][kCands];      // [is_double][cand]
     double ovh[2][kCands];
     double bw;                  // summation bandwidth, bytes/s
     double launch_ms;           // fixed cost per launch
     int cus;
+    double teff[kTiles];        // rate of each tile's kernel relative to the 128 x 128 one (whole tiles)
 };
 
 std::mutex g_tuner_mu;
@@ -47,7 +54,8 @@ TunerParams g_tp = {
     // defaults: fmmhip_tuner_calibrate on an MI355X (FP32 row, then FP64)
     {{0.71, 0.69, 0.66}, {0.88, 0.87, 0.83}},
     {{4.0, 4.0, 4.0}, {4.0, 4.0, 4.0}},
-    3.9e12, 0.01, 256};
+    3.9e12, 0.01, 256,
+    {1.0, 0.92}};                // 64 x 64 tiles: 63.3 against 68.7 TFLOPS at 16384^3 (FP64, measured)
 
 const double kPeak[2] = {157.3e12, 78.6e12};   // FP32, FP64 MFMA peaks (FLOP/s)
 
@@ -56,17 +64,18 @@ struct CandShape {
 };
 const CandShape kShape[kCands] = {{1, 1, 1, 1}, {2, 2, 2, 7}, {4, 4, 4, 49}};
 
-double est_ms(const TunerParams& tp, int is_double, int cand, long m, long n, long k, bool lone)
+double est_ms(const TunerParams& tp, int is_double, int cand, long m, long n, long k, bool lone, int tile = 0)
 {
     const CandShape& c = kShape[cand];
     const double lm = (double)((m + c.mt - 1) / c.mt), lk = (double)((k + c.kt - 1) / c.kt),
                  ln = (double)((n + c.nt - 1) / c.nt);
-    const double fl = 2.0 * c.R * lm * lk * ln;
+    const double T = (double)fmmk::tile_size(kTileShape[tile]);
+    const double tiles = std::ceil(lm / T) * std::ceil(ln / T);
+    const double fl = 2.0 * c.R * lk * tiles * T * T;      // the tile grid's flops, padding included
     const double nk = std::ceil(lk / fmmk::BK);
-    double eff = tp.eff[is_double][cand] * nk / (nk + tp.ovh[is_double][cand]);
+    double eff = tp.eff[is_double][cand] * tp.teff[tile] * nk / (nk + tp.ovh[is_double][cand]);
     if (lone) {
-        const double tiles = std::ceil(lm / fmmk::BM) * std::ceil(ln / fmmk::BN);
-        const double slots = 2.0 * tp.cus;
+        const double slots = (double)kTileWgPerCu[tile] * tp.cus;
         const double waves = std::ceil(tiles / slots);
         eff *= tiles / (waves * slots);
     }
@@ -84,24 +93,46 @@ double est_ms(const TunerParams& tp, int is_double, int cand, long m, long n, lo
 // (workspace, a summation launch, long work items) than a wrong classical one.
 constexpr double kFmmMargin = 0.95;
 
-int choose(int is_double, long m, long n, long k, bool lone, double* best_ms)
+// The faster tile for candidate `cand` (ties keep the 128 x 128 tile).  A
+// lone classical call with at most half the CUs covered by 128 x 128 tiles
+// splits k instead (one-problem path), which fills the device better.
+int best_tile(const TunerParams& tp, int is_double, int cand, long m, long n, long k, bool lone, double* ms)
+{
+    int best = 0;
+    double bt = est_ms(tp, is_double, cand, m, n, k, lone, 0);
+    const long T0 = fmmk::tile_size(kTileShape[0]);
+    if (lone && cand == 0 && 2 * ((m + T0 - 1) / T0) * ((n + T0 - 1) / T0) <= tp.cus) {
+        if (ms) *ms = bt;
+        return 0;
+    }
+    for (int t = 1; t < kTiles; t++) {
+        const double e = est_ms(tp, is_double, cand, m, n, k, lone, t);
+        if (e < bt) { bt = e; best = t; }
+    }
+    if (ms) *ms = bt;
+    return best;
+}
+
+int choose(int is_double, long m, long n, long k, bool lone, double* best_ms, int* tile_out = nullptr)
 {
     TunerParams tp;
     {
         std::lock_guard<std::mutex> lk(g_tuner_mu);
         tp = g_tp;
     }
-    int best = 0;
+    int best = 0, best_t = 0;
     double bt = 1e300, tmin = 1e300;   // the chosen estimate; the least over every simpler candidate
     for (int c = 0; c < kCands; c++) {
         // a level needs leaves of at least two tiles and k of 8 BK-steps
         const CandShape& s = kShape[c];
         if (c > 0 && (m < 2L * fmmk::BM * s.mt || n < 2L * fmmk::BN * s.nt || k < 8L * fmmk::BK * s.kt)) continue;
-        const double t = est_ms(tp, is_double, c, m, n, k, lone);
-        if (c == 0 || t < kFmmMargin * tmin) { bt = t; best = c; }
+        double t = 0.0;
+        const int tile = best_tile(tp, is_double, c, m, n, k, lone, &t);
+        if (c == 0 || t < kFmmMargin * tmin) { bt = t; best = c; best_t = tile; }
         tmin = std::min(tmin, t);
     }
     if (best_ms) *best_ms = bt;
+    if (tile_out) *tile_out = best_t;
     return best;
 }
 
@@ -128,6 +159,16 @@ extern "C" int fmmhip_tuner_choose(int is_double, long m, long n, long k, int lo
     return FMMHIP_SUCCESS;
 }
 
+extern "C" int fmmhip_tuner_choose_tile(int is_double, long m, long n, long k, int lone, int* cand, int* tile,
+                                        double* est)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || !cand || !tile) return FMMHIP_ERR_INVALID;
+    int t = 0;
+    *cand = choose(is_double ? 1 : 0, m, n, k, lone != 0, est, &t);
+    *tile = fmmk::tile_size(kTileShape[t]);
+    return FMMHIP_SUCCESS;
+}
+
 extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, long k, int lone, double* est)
 {
     if (cand < 0 || cand >= kCands || !est) return FMMHIP_ERR_INVALID;
@@ -136,7 +177,7 @@ extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, lo
         std::lock_guard<std::mutex> lk(g_tuner_mu);
         tp = g_tp;
     }
-    *est = est_ms(tp, is_double ? 1 : 0, cand, m, n, k, lone != 0);
+    (void)best_tile(tp, is_double ? 1 : 0, cand, m, n, k, lone != 0, est);   // on the candidate's faster tile
     return FMMHIP_SUCCESS;
 }
 
@@ -239,6 +280,8 @@ extern "C" int fmmhip_tuner_save(const char* path)
     for (int d = 0; d < 2; d++)
         for (int c = 0; c < kCands; c++) fprintf(f, "cand %d %s %.6f %.6f\n", c, d ? "f64" : "f32", tp.eff[d][c], tp.ovh[d][c]);
     fprintf(f, "bw %.6e\nlaunch_ms %.6f\ncus %d\n", tp.bw, tp.launch_ms, tp.cus);
+    fprintf(f, "# relative rate of the 128 x 128 and 64 x 64 tile kernels\ntile_eff %.6f %.6f\n", tp.teff[0],
+            tp.teff[1]);
     fclose(f);
     return FMMHIP_SUCCESS;
 }
@@ -271,6 +314,9 @@ extern "C" int fmmhip_tuner_load(const char* path)
             tp.launch_ms = e;
         } else if (sscanf(line, "cus %d", &c) == 1 && c > 0) {
             tp.cus = c;
+        } else if (sscanf(line, "tile_eff %lf %lf", &e, &o) == 2 && e > 0 && e <= 1.5 && o > 0 && o <= 1.5) {
+            tp.teff[0] = e;
+            tp.teff[1] = o;
         }
     }
     fclose(f);
@@ -294,13 +340,15 @@ struct fmmhip_batch_s {
         void* C;
         long ldc;
         int force;              // -1 tuner, else candidate
+        int force_tile;         // -1 tuner, else tile index
         int cand;               // chosen
+        int tile;               // chosen tile index
     };
     std::vector<Prob> probs;
     std::vector<Prepared> prep;
     // the batch's own algorithms: their descriptor caches never evict, so
     // every pointer captured in the graph stays valid for the batch's life
-    fmmhip_alg_t algs[kCands] = {nullptr, nullptr, nullptr};
+    fmmhip_alg_t algs[kTiles][kCands] = {};
     hipStream_t stream = nullptr;
     void* ws = nullptr;
     void* dev = nullptr;        // KArgs, items, segments, block offsets, heads
@@ -308,7 +356,7 @@ struct fmmhip_batch_s {
     hipGraphExec_t exec = nullptr;
     bool planned = false;
     double est_ms = 0.0;
-    long items[2] = {0, 0};
+    long items[2] = {0, 0};     // work items per datatype (both tiles)
 
     void release()
     {
@@ -320,20 +368,27 @@ struct fmmhip_batch_s {
         if (ws) (void)hipFree(ws);
         if (dev) (void)hipFree(dev);
         ws = dev = nullptr;
-        for (int c = 0; c < kCands; c++) {
-            fmmhip_alg_destroy(algs[c]);
-            algs[c] = nullptr;
-        }
+        for (int t = 0; t < kTiles; t++)
+            for (int c = 0; c < kCands; c++) {
+                fmmhip_alg_destroy(algs[t][c]);
+                algs[t][c] = nullptr;
+            }
         planned = false;
     }
 
     int make_algs()
     {
-        int rc = fmmhip_alg_create_classical(&algs[0]);
-        if (!rc) rc = fmmhip_alg_create(&algs[1], &STRASSEN_FMM, 1);
-        fmm_t two[2] = {STRASSEN_FMM, STRASSEN_FMM};
-        if (!rc) rc = fmmhip_alg_create(&algs[2], two, 2);
-        for (int c = 0; c < kCands && !rc; c++) algs[c]->keep_all = 1;
+        int rc = 0;
+        for (int t = 0; t < kTiles && !rc; t++) {
+            rc = fmmhip_alg_create_classical(&algs[t][0]);
+            if (!rc) rc = fmmhip_alg_create(&algs[t][1], &STRASSEN_FMM, 1);
+            fmm_t two[2] = {STRASSEN_FMM, STRASSEN_FMM};
+            if (!rc) rc = fmmhip_alg_create(&algs[t][2], two, 2);
+            for (int c = 0; c < kCands && !rc; c++) {
+                algs[t][c]->keep_all = 1;
+                rc = fmmhip_alg_set_shape(algs[t][c], kTileShape[t]);
+            }
+        }
         return rc;
     }
 };
@@ -367,7 +422,7 @@ static int batch_add(fmmhip_batch_t b, int dbl, long m, long n, long k, double a
         ldc < (m > 1 ? m : 1) || force < -1 || force >= kCands)
         return FMMHIP_ERR_INVALID;
     b->release();
-    fmmhip_batch_s::Prob p{dbl, m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, force, -1};
+    fmmhip_batch_s::Prob p{dbl, m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, force, -1, -1, 0};
     b->probs.push_back(p);
     return (int)b->probs.size() - 1 >= 0 ? FMMHIP_SUCCESS : FMMHIP_ERR_INVALID;
 }
@@ -399,16 +454,27 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     for (size_t i = 0; i < np && !rc; i++) {
         fmmhip_batch_s::Prob& p = b->probs[i];
         double est = 0.0;
-        if (p.force >= 0) {
-            p.cand = p.force;
-            if (p.m > 0 && p.n > 0 && p.k > 0) fmmhip_tuner_estimate(p.dbl, p.cand, p.m, p.n, p.k, 0, &est);
-        } else if (p.m > 0 && p.n > 0 && p.k > 0) {
-            p.cand = choose(p.dbl, p.m, p.n, p.k, false, &est);
-        } else {
-            p.cand = 0;
+        TunerParams tp;
+        {
+            std::lock_guard<std::mutex> lk(g_tuner_mu);
+            tp = g_tp;
+        }
+        const bool work = p.m > 0 && p.n > 0 && p.k > 0;
+        p.cand = 0;
+        p.tile = 0;
+        if (work && p.force < 0 && p.force_tile < 0) {
+            p.cand = choose(p.dbl, p.m, p.n, p.k, false, &est, &p.tile);
+        } else if (work) {
+            p.cand = p.force >= 0 ? p.force : choose(p.dbl, p.m, p.n, p.k, false, nullptr);
+            if (p.force_tile >= 0) {
+                p.tile = p.force_tile;
+                est = est_ms(tp, p.dbl, p.cand, p.m, p.n, p.k, false, p.tile);
+            } else {
+                p.tile = best_tile(tp, p.dbl, p.cand, p.m, p.n, p.k, false, &est);
+            }
         }
         b->est_ms += est;
-        rc = prepare_call(b->algs[p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1, p.lda,
+        rc = prepare_call(b->algs[p.tile][p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1, p.lda,
                           p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? 1 : 0, 1, b->stream, &b->prep[i]);
         // the batched kernel is instantiated for column-major operands only
         if (!rc && !b->prep[i].nothing && !b->prep[i].scale_only &&
@@ -429,14 +495,17 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         for (size_t i = 0; i < np; i++)
             if (b->prep[i].presum) bind_workspace(&b->prep[i], (char*)b->ws + ws_off[i]);
     }
-    // 2. per datatype: kernel arguments, work items (costliest first),
-    //    summation segments and their block offsets
-    std::vector<KArgs> ka[2];
-    std::vector<BatchItem> items[2];
-    std::vector<double> cost[2];
+    // 2. per datatype and tile (group g = 2 * tile + datatype): kernel
+    //    arguments and work items (costliest first); per datatype: summation
+    //    segments and their block offsets
+    constexpr int kGroups = 2 * kTiles;
+    std::vector<KArgs> ka[kGroups];
+    std::vector<BatchItem> items[kGroups];
+    std::vector<double> cost[kGroups];
     std::vector<fmmhip_presum_seg_t> segs[2];
     std::vector<int> boff[2];
-    bool vec[2] = {true, true};
+    bool vec[kGroups];
+    for (int g = 0; g < kGroups; g++) vec[g] = true;
     std::vector<size_t> scale_only;
     bool parts[2] = {true, true};     // part-staged summation for every segment of the datatype
     for (int d = 0; d < 2; d++) boff[d].push_back(0);
@@ -445,13 +514,14 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         const int d = b->probs[i].dbl;
         if (P.nothing) continue;
         if (P.scale_only) { scale_only.push_back(i); continue; }
-        const int pi = (int)ka[d].size();
-        ka[d].push_back(P.ka);
-        vec[d] = vec[d] && P.ka.va && P.ka.vb;
+        const int g = 2 * (P.shape == 2 ? 1 : 0) + d;
+        const int pi = (int)ka[g].size();
+        ka[g].push_back(P.ka);
+        vec[g] = vec[g] && P.ka.va && P.ka.vb;
         const double c = (double)P.ka.R * (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
         for (long t = 0; t < P.nwg; t++) {
-            items[d].push_back(BatchItem{pi, (int)t});
-            cost[d].push_back(c);
+            items[g].push_back(BatchItem{pi, (int)t});
+            cost[g].push_back(c);
         }
         if (P.presum) {
             for (const fmmhip_presum_seg_t* sg : {&P.segA, &P.segB}) {
@@ -460,14 +530,15 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             }
         }
     }
-    for (int d = 0; d < 2; d++) {
-        std::vector<size_t> order(items[d].size());
+    b->items[0] = b->items[1] = 0;
+    for (int g = 0; g < kGroups; g++) {
+        std::vector<size_t> order(items[g].size());
         for (size_t j = 0; j < order.size(); j++) order[j] = j;
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[d][x] > cost[d][y]; });
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[g][x] > cost[g][y]; });
         std::vector<BatchItem> sorted(order.size());
-        for (size_t j = 0; j < order.size(); j++) sorted[j] = items[d][order[j]];
-        items[d].swap(sorted);
-        b->items[d] = (long)items[d].size();
+        for (size_t j = 0; j < order.size(); j++) sorted[j] = items[g][order[j]];
+        items[g].swap(sorted);
+        b->items[g % 2] += (long)items[g].size();
     }
     // block ranges of the summation segments: part-staged units are 256
     // elements along c (32 per block), entry-driven ones pairs (4096 per block)
@@ -477,24 +548,26 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             const long per = parts[d] ? 32 : 4096;
             boff[d].push_back(boff[d].back() + (int)std::max(1L, (work + per - 1) / per));
         }
-    // 3. one device block: [ka0 | ka1 | items0 | items1 | segs0 | segs1 | boff0 | boff1 | head0 head1]
+    // 3. one device block: [ka per group | items per group | segs0 | segs1 | boff0 | boff1 | heads]
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) & ~(size_t)255; return o; };
-    size_t o_ka[2], o_it[2], o_sg[2], o_bo[2];
-    for (int d = 0; d < 2; d++) o_ka[d] = take(sizeof(KArgs) * (ka[d].size() + 1));
-    for (int d = 0; d < 2; d++) o_it[d] = take(sizeof(BatchItem) * (items[d].size() + 1));
+    size_t o_ka[kGroups], o_it[kGroups], o_sg[2], o_bo[2];
+    for (int g = 0; g < kGroups; g++) o_ka[g] = take(sizeof(KArgs) * (ka[g].size() + 1));
+    for (int g = 0; g < kGroups; g++) o_it[g] = take(sizeof(BatchItem) * (items[g].size() + 1));
     for (int d = 0; d < 2; d++) o_sg[d] = take(sizeof(fmmhip_presum_seg_t) * (segs[d].size() + 1));
     for (int d = 0; d < 2; d++) o_bo[d] = take(sizeof(int) * (boff[d].size() + 1));
-    const size_t o_head = take(2 * sizeof(int));
+    const size_t o_head = take(kGroups * sizeof(int));
     if (hipMalloc(&b->dev, off) != hipSuccess) { (void)hipGetLastError(); b->dev = nullptr; return FMMHIP_ERR_NOMEM; }
     char* dv = (char*)b->dev;
-    for (int d = 0; d < 2; d++) {
-        if (!ka[d].empty() &&
-            hipMemcpy(dv + o_ka[d], ka[d].data(), sizeof(KArgs) * ka[d].size(), hipMemcpyHostToDevice) != hipSuccess)
+    for (int g = 0; g < kGroups; g++) {
+        if (!ka[g].empty() &&
+            hipMemcpy(dv + o_ka[g], ka[g].data(), sizeof(KArgs) * ka[g].size(), hipMemcpyHostToDevice) != hipSuccess)
             return FMMHIP_ERR_HIP;
-        if (!items[d].empty() && hipMemcpy(dv + o_it[d], items[d].data(), sizeof(BatchItem) * items[d].size(),
+        if (!items[g].empty() && hipMemcpy(dv + o_it[g], items[g].data(), sizeof(BatchItem) * items[g].size(),
                                            hipMemcpyHostToDevice) != hipSuccess)
             return FMMHIP_ERR_HIP;
+    }
+    for (int d = 0; d < 2; d++) {
         if (!segs[d].empty() && hipMemcpy(dv + o_sg[d], segs[d].data(), sizeof(fmmhip_presum_seg_t) * segs[d].size(),
                                           hipMemcpyHostToDevice) != hipSuccess)
             return FMMHIP_ERR_HIP;
@@ -502,7 +575,7 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             return FMMHIP_ERR_HIP;
     }
     if (hipStreamSynchronize(b->stream) != hipSuccess) return FMMHIP_ERR_HIP;   // descriptor uploads
-    // 4. capture: summation, then the persistent kernel, per datatype
+    // 4. capture: per datatype, the summation, then one persistent kernel per tile
     int cus = 256;
     fmmhip_device_cu_count(&cus);
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FMMHIP_ERR_HIP;
@@ -516,15 +589,17 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         if (!segs[d].empty())
             rc = fmmhip_presum_batch(d, (const fmmhip_presum_seg_t*)(dv + o_sg[d]), (int)segs[d].size(),
                                      (const int*)(dv + o_bo[d]), boff[d].back(), parts[d] ? 1 : 0, b->stream);
-        if (!rc && !items[d].empty()) {
-            const int nwg = (int)std::min<long>((long)items[d].size(), 2L * cus);
-            int* head = (int*)(dv + o_head) + d;
+        for (int t = 0; t < kTiles && !rc; t++) {
+            const int g = 2 * t + d;
+            if (items[g].empty()) continue;
+            const int nwg = (int)std::min<long>((long)items[g].size(), (long)kTileWgPerCu[t] * cus);
+            int* head = (int*)(dv + o_head) + g;
             if (d)
-                rc = fmmk_launch_batch_d64((const KArgs*)(dv + o_ka[d]), (const BatchItem*)(dv + o_it[d]),
-                                           (int)items[d].size(), head, vec[d], nwg, b->stream);
+                rc = fmmk_launch_batch_d64((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]),
+                                           (int)items[g].size(), head, vec[g], kTileShape[t], nwg, b->stream);
             else
-                rc = fmmk_launch_batch_s32((const KArgs*)(dv + o_ka[d]), (const BatchItem*)(dv + o_it[d]),
-                                           (int)items[d].size(), head, vec[d], nwg, b->stream);
+                rc = fmmk_launch_batch_s32((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]),
+                                           (int)items[g].size(), head, vec[g], kTileShape[t], nwg, b->stream);
         }
     }
     hipGraph_t g = nullptr;
@@ -567,5 +642,24 @@ extern "C" int fmmhip_batch_choice(fmmhip_batch_t b, long i, int* cand)
 {
     if (!b || i < 0 || (size_t)i >= b->probs.size() || !cand) return FMMHIP_ERR_INVALID;
     *cand = b->probs[i].cand;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_batch_set_tile(fmmhip_batch_t b, long i, int tile)
+{
+    if (!b || i < 0 || (size_t)i >= b->probs.size() || !(tile == 0 || tile == 64 || tile == 128))
+        return FMMHIP_ERR_INVALID;
+    b->release();
+    b->probs[i].force_tile = tile == 0 ? -1 : tile == 64 ? 1 : 0;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_batch_tile(fmmhip_batch_t b, long i, int* tile)
+{
+    if (!b || i < 0 || (size_t)i >= b->probs.size() || !tile) return FMMHIP_ERR_INVALID;
+    if (!b->planned) return FMMHIP_ERR_INVALID;
+    const fmmhip_batch_s::Prob& p = b->probs[i];
+    // a problem the 64 x 64 kernel cannot take runs on the 128 x 128 one
+    *tile = fmmk::tile_size(b->prep[i].nothing || b->prep[i].scale_only ? kTileShape[p.tile] : b->prep[i].shape);
     return FMMHIP_SUCCESS;
 }

@@ -876,8 +876,6 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     ka.lm = (int)g.lm; ka.lk = (int)g.lk; ka.ln = (int)g.ln;
     ka.min_cm = (int)(m / alg->mt); ka.min_cn = (int)(n / alg->nt);
     ka.R = alg->R;
-    ka.tiles_m = (int)((g.lm + BM - 1) / BM);
-    ka.tiles_n = (int)((g.ln + BN - 1) / BN);
     ka.alpha = alpha;
     ka.beta = beta;
     int shape = alg->shape;
@@ -924,10 +922,16 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.vb = pairs_aligned(B, esz, ka.ldb, dv.fused_ent_h, alg->nnz_u, alg->nnz_u + alg->nnz_v);
         if (shape < 0) shape = multi ? 1 : 0;
     }
+    // the 64 x 64 tile shape is instantiated for column-major A, B, C only
+    if (shape == 2 && !(a_mn && !b_mn && rsC == 1)) shape = 0;
     P->shape = shape;
     P->a_mn = a_mn;
     P->b_mn = b_mn;
     P->c_col = rsC == 1;
+    const int T = fmmk::tile_size(shape);
+    const int wg_per_cu = shape == 2 ? 4 : 2;      // resident workgroups per CU (LDS and registers)
+    ka.tiles_m = (int)((g.lm + T - 1) / T);
+    ka.tiles_n = (int)((g.ln + T - 1) / T);
     const long tiles = (long)ka.tiles_m * ka.tiles_n;
     // split: group g of every tile scatters into temporary g (the
     // temporaries span C's whole extent, gaps between columns included);
@@ -939,7 +943,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     // of one tile per CU gains too little to pay for the temporaries
     int cus = 256;
     fmmhip_device_cu_count(&cus);
-    const bool kspl = alg->R == 1 && a_mn && !b_mn && rsC == 1 && 2 * tiles <= cus;
+    const bool kspl = alg->R == 1 && a_mn && !b_mn && rsC == 1 && 2 * tiles <= cus && shape != 2;
     const long ksteps = (g.lk + fmmk::BK - 1) / fmmk::BK;
     if (wmat) {
         // every product writes its own slot: the groups need no copies of C
@@ -957,7 +961,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.ptr = dv.mat_ptr;
         ka.ent = dv.mat_ent;
         P->c_col = !sl.swap;
-        ka.groups = split > 0 ? std::min(split, alg->R) : mat_groups(alg->R, tiles, 2 * cus);
+        ka.groups = split > 0 ? std::min(split, alg->R) : mat_groups(alg->R, tiles, wg_per_cu * cus);
         ka.ksplit = false;
         ka.gbit = 0;
         ka.tmp = ka.C;           // group g's Cb: the slots (no per-group copy)
@@ -1001,7 +1005,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     }
     ka.groups = split_groups(split, alg->R == 1 ? (kspl ? (int)std::min(ksteps / 8, 8L) : 1) : alg->R, tiles,
                              2.0 * alg->R * (double)g.lm * (double)g.lk * (double)g.ln, (double)esz * c_span,
-                             kind == Kind::D64, 0.0, 2);
+                             kind == Kind::D64, 0.0, wg_per_cu);
     ka.ksplit = kspl && ka.groups > 1;
     ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
     ka.tmp = nullptr;
@@ -1341,7 +1345,7 @@ fmmhip_alg_t classical_alg()
 
 extern "C" int fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape)
 {
-    if (!alg || shape < -1 || shape > 1) return FMMHIP_ERR_INVALID;
+    if (!alg || shape < -1 || shape > 2) return FMMHIP_ERR_INVALID;
     alg->shape = shape;
     return FMMHIP_SUCCESS;
 }

@@ -8,7 +8,7 @@ int fmmk_launch_d64(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool
 }
 
 int fmmk_launch_batch_d64(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
-                          int nwg, hipStream_t st)
+                          int shape, int nwg, hipStream_t st)
 {
-    return fmmk::launch_batch_typed<double, double, double>(probs, items, nitems, head, vec, nwg, st);
+    return fmmk::launch_batch_typed<double, double, double>(probs, items, nitems, head, vec, shape, nwg, st);
 }

@@ -21,7 +21,7 @@
 //                applied afterwards by fmmhip_wsum.hip)
 //   R loop       frame/base/bli_strassen_ab.c:248-294 -- inside the kernel
 //
-// Work decomposition: one workgroup per BM x BN tile position of the leaf,
+// Work decomposition: one workgroup per TILE x TILE position of the leaf,
 // looping over all R products.  It is the only writer of that tile position
 // in every C part, so the read-modify-write of C is race-free without
 // atomics and deterministic.  The first product that writes a part applies
@@ -40,47 +40,57 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d2_t __attribute__((ext_vector_type(2)));
 typedef float f2_t __attribute__((ext_vector_type(2)));
 
-constexpr int BM = 128;          // tile rows (leaf M)
-constexpr int BN = 128;          // tile cols (leaf N)
+constexpr int BM = 128;          // tile rows and columns of the default shapes (0, 1)
+constexpr int BN = 128;
 constexpr int BK = 16;           // k per LDS stage
 // LDS tiles are kept [BK][row] (mn innermost): rows of 16 lanes read 128
 // contiguous bytes.  The transposing store of a k-contiguous operand writes
 // 8 even k rows of one mn column per 8 lanes; the row stride spreads them
-// over the banks: BM + 1 doubles (8 distinct bank pairs per 8 lanes) and
-// BM + 2 floats (32 distinct banks per 32 lanes).  A stride of BM + 16 put
-// all 8 rows in one bank (8-way conflicts, 6% of the FP64 kernel's time).
-template <typename TL>
-__device__ __host__ constexpr int lds_row() { return sizeof(TL) == 8 ? BM + 1 : BM + 2; }
-template <typename TL>
-__device__ __host__ constexpr int lds_tile() { return BK * lds_row<TL>(); }   // elements per tile buffer
+// over the banks: T + 1 doubles (8 distinct bank pairs per 8 lanes) and
+// T + 2 floats (32 distinct banks per 32 lanes), T the tile's rows.  A
+// stride of T + 16 put all 8 rows in one bank (8-way conflicts, 6% of the
+// FP64 kernel's time).
+template <typename TL, int T>
+__device__ __host__ constexpr int lds_row() { return sizeof(TL) == 8 ? T + 1 : T + 2; }
+template <typename TL, int T>
+__device__ __host__ constexpr int lds_tile() { return BK * lds_row<TL, T>(); }   // elements per tile buffer
 
 // Element (mn, k) of an LDS tile.
-template <typename TL>
-__device__ __forceinline__ constexpr int lds_at(int mn, int k) { return k * lds_row<TL>() + mn; }
+template <typename TL, int T>
+__device__ __forceinline__ constexpr int lds_at(int mn, int k) { return k * lds_row<TL, T>() + mn; }
 
-// Launch shape: NT threads as a WAVES_M x WAVES_N grid of waves, OCC waves
-// per SIMD.
+// Launch shape: NT threads as a WAVES_M x WAVES_N grid of waves over a
+// TILE x TILE tile of C, OCC waves per SIMD.
 // MULTI: the parts of a product after the first are interleaved with the
 // MFMA sub-blocks (one extra register set in flight); otherwise they are
 // loaded after the MFMA block.
-//   Shape 0: 512 threads, 64 x 32 per wave, two workgroups per CU, single
-//   Shape 1: 256 threads, 64 x 64 per wave, two workgroups per CU, multi
+//   Shape 0: 128 x 128 tile, 512 threads, 64 x 32 per wave, two workgroups per CU, single
+//   Shape 1: 128 x 128 tile, 256 threads, 64 x 64 per wave, two workgroups per CU, multi
+//   Shape 2:  64 x  64 tile, 256 threads, 32 x 32 per wave, four workgroups per CU, single
+//            (0.92x shape 0's rate on full tiles, measured; no padding waste
+//            on dimensions that are odd multiples of 64)
 template <int SHAPE> struct Cfg;
-template <> struct Cfg<0> { static constexpr int NT = 512, WAVES_M = 2, WAVES_N = 4, OCC = 4; static constexpr bool MULTI = false; };
-template <> struct Cfg<1> { static constexpr int NT = 256, WAVES_M = 2, WAVES_N = 2, OCC = 2; static constexpr bool MULTI = true; };
+template <> struct Cfg<0> { static constexpr int NT = 512, WAVES_M = 2, WAVES_N = 4, OCC = 4, TILE = 128; static constexpr bool MULTI = false; };
+template <> struct Cfg<1> { static constexpr int NT = 256, WAVES_M = 2, WAVES_N = 2, OCC = 2, TILE = 128; static constexpr bool MULTI = true; };
+template <> struct Cfg<2> { static constexpr int NT = 256, WAVES_M = 2, WAVES_N = 2, OCC = 4, TILE = 64; static constexpr bool MULTI = false; };
+
+// Tile rows (= columns) of a launch shape, on the host.
+inline int tile_size(int shape) { return shape == 2 ? Cfg<2>::TILE : Cfg<0>::TILE; }
 
 template <int SHAPE> struct Geo {
     static constexpr int NT = Cfg<SHAPE>::NT;
-    static constexpr int WTM = BM / Cfg<SHAPE>::WAVES_M;
-    static constexpr int WTN = BN / Cfg<SHAPE>::WAVES_N;
+    static constexpr int T = Cfg<SHAPE>::TILE;
+    static constexpr int WTM = T / Cfg<SHAPE>::WAVES_M;
+    static constexpr int WTN = T / Cfg<SHAPE>::WAVES_N;
     static constexpr int FI = WTM / 16;            // 16x16 MFMA blocks per wave along M
     static constexpr int FJ = WTN / 16;            // ... along N
-    static constexpr int PAIRS = BM * BK / 2 / NT; // staged pairs per thread per operand
-    static constexpr int MN_PAIRS = BM / 2;        // pairs along mn in one k row
+    static constexpr int PAIRS = T * BK / 2 / NT;  // staged pairs per thread per operand
+    static constexpr int MN_PAIRS = T / 2;         // pairs along mn in one k row
     static constexpr int K_ROWS = NT / MN_PAIRS;   // k rows per pass (mn-contiguous)
     static constexpr int K_PAIRS = BK / 2;         // pairs along k in one mn column
     static constexpr int MN_COLS = NT / K_PAIRS;   // mn columns per pass (k-contiguous)
-    static_assert(PAIRS * NT * 2 == BM * BK, "staging covers the tile");
+    static_assert(PAIRS * NT * 2 == T * BK, "staging covers the tile");
+    static_assert(PAIRS * K_ROWS == BK && PAIRS * MN_COLS == T, "staging passes cover the tile");
 };
 
 struct KArgs {
@@ -138,7 +148,7 @@ template <typename TI> struct Pair;
 template <> struct Pair<double> { typedef d2_t t; };
 template <> struct Pair<float> { typedef f2_t t; };
 
-// Staging of one operand tile (BM or BN along "mn", BK along k) for one
+// Staging of one operand tile (TILE along "mn", BK along k) for one
 // k-step: the weighted sum over the product's nonzero parts.  The raw values
 // of up to NE parts are loaded before the MFMA block of the current k-step
 // (their loads are in flight while the matrix cores run) and combined after
@@ -259,12 +269,13 @@ struct Stage {
     template <typename TL>
     __device__ __forceinline__ void store(TL* lds) const
     {
-        TL* q = lds + lds_at<TL>(lmn0, lk0);
+        constexpr int T = Geo<SHAPE>::T;
+        TL* q = lds + lds_at<TL, T>(lmn0, lk0);
 #pragma unroll
         for (int s = 0; s < PAIRS; s++) {
-            TL* qs = q + lds_at<TL>(dmn(s), dk(s));
+            TL* qs = q + lds_at<TL, T>(dmn(s), dk(s));
             qs[0] = (TL)sum[s][0];
-            qs[lds_at<TL>(MN_CONTIG ? 1 : 0, MN_CONTIG ? 0 : 1)] = (TL)sum[s][1];   // next along the contiguous dim
+            qs[lds_at<TL, T>(MN_CONTIG ? 1 : 0, MN_CONTIG ? 0 : 1)] = (TL)sum[s][1];   // next along the contiguous dim
         }
     }
 };
@@ -321,16 +332,18 @@ __device__ __forceinline__ void epi_block(TO* __restrict__ cb, int rs32, int ero
         }
 }
 
-// One BM x BN tile position (tm, tn) of the leaf, all R products: the
+// One TILE x TILE position (tm, tn) of the leaf, all R products: the
 // body of both the one-problem kernel and the batched persistent kernel.
 // sA/sB are the workgroup's double-buffered LDS tiles.
 // KSPL: the groups split the k-steps (a.ksplit; instantiated apart, so that
 // the other kernels keep their registers).
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC, bool A_MN, bool B_MN, bool C_COL,
           bool KSPL = false>
-__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int grp, TL (*sA)[lds_tile<TL>()],
-                                          TL (*sB)[lds_tile<TL>()])
+__device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int grp,
+                                          TL (*sA)[lds_tile<TL, Cfg<SHAPE>::TILE>()],
+                                          TL (*sB)[lds_tile<TL, Cfg<SHAPE>::TILE>()])
 {
+    constexpr int BM = Cfg<SHAPE>::TILE, BN = Cfg<SHAPE>::TILE;   // this shape's tile
     constexpr int WAVES_M = Cfg<SHAPE>::WAVES_M;
     constexpr bool MULTI = Cfg<SHAPE>::MULTI;
     constexpr int WTM = Geo<SHAPE>::WTM, WTN = Geo<SHAPE>::WTN, FI = Geo<SHAPE>::FI, FJ = Geo<SHAPE>::FJ;
@@ -448,9 +461,9 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
                 const int kk = 4 * ks + (lane >> 4);
                 TL fa[FI], fb[FJ];
 #pragma unroll
-                for (int i = 0; i < FI; i++) fa[i] = la[lds_at<TL>(wm + 16 * i + (lane & 15), kk)];
+                for (int i = 0; i < FI; i++) fa[i] = la[lds_at<TL, BM>(wm + 16 * i + (lane & 15), kk)];
 #pragma unroll
-                for (int j = 0; j < FJ; j++) fb[j] = lb[lds_at<TL>(wn + 16 * j + (lane & 15), kk)];
+                for (int j = 0; j < FJ; j++) fb[j] = lb[lds_at<TL, BN>(wn + 16 * j + (lane & 15), kk)];
 #pragma unroll
                 for (int i = 0; i < FI; i++)
 #pragma unroll
@@ -535,8 +548,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>
                           amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
 void fmm_gemm_kernel(KArgs a)
 {
-    __shared__ TL sA[2][lds_tile<TL>()];
-    __shared__ TL sB[2][lds_tile<TL>()];
+    __shared__ TL sA[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
+    __shared__ TL sB[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
     // blocks [g*tiles, (g+1)*tiles) are product group g
     const int tiles = a.tiles_m * a.tiles_n;
     const int grp = (int)blockIdx.x / tiles;
@@ -560,8 +573,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>
 void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items, int nitems,
                       int* __restrict__ head)
 {
-    __shared__ TL sA[2][lds_tile<TL>()];
-    __shared__ TL sB[2][lds_tile<TL>()];
+    __shared__ TL sA[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
+    __shared__ TL sB[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
     __shared__ int slot;
     for (;;) {
         if (threadIdx.x == 0) slot = atomicAdd(head, 1);
@@ -612,10 +625,22 @@ int launch_layouts(const KArgs& ka, bool a_mn, bool b_mn, bool c_col, int nwg, h
 }
 
 // Vector loads need both operands' pairs 16-byte aligned (ka.va && ka.vb).
+// Shape 2 (64 x 64 tiles) is instantiated for column-major A, B and C
+// without the k-split only.
 template <typename TI, typename TL, typename TO>
 int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st)
 {
     const bool vec = ka.va && ka.vb;
+    if (shape == 2) {
+        if (ka.ksplit || !(a_mn && !b_mn && c_col)) return FMMHIP_ERR_UNSUPPORTED;
+        if (vec)
+            hipLaunchKernelGGL((fmm_gemm_kernel<TI, TL, TO, 2, true, true, false, true>), dim3(nwg), dim3(Cfg<2>::NT),
+                               0, st, ka);
+        else
+            hipLaunchKernelGGL((fmm_gemm_kernel<TI, TL, TO, 2, false, true, false, true>), dim3(nwg), dim3(Cfg<2>::NT),
+                               0, st, ka);
+        return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+    }
     if (shape == 1) {
         if (vec) return launch_layouts<TI, TL, TO, 1, true>(ka, a_mn, b_mn, c_col, nwg, st);
         return launch_layouts<TI, TL, TO, 1, false>(ka, a_mn, b_mn, c_col, nwg, st);
@@ -625,18 +650,24 @@ int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, i
 }
 
 // Batched launch (column-major A, B, C): reset the queue head, then the
-// persistent kernel on `nwg` workgroups.
+// persistent kernel of launch shape `shape` (0: 128 x 128 tiles, 2: 64 x 64)
+// on `nwg` workgroups.
 template <typename TI, typename TL, typename TO>
-int launch_batch_typed(const KArgs* probs, const BatchItem* items, int nitems, int* head, bool vec, int nwg,
+int launch_batch_typed(const KArgs* probs, const BatchItem* items, int nitems, int* head, bool vec, int shape, int nwg,
                        hipStream_t st)
 {
     hipLaunchKernelGGL(batch_reset_kernel<0>, dim3(1), dim3(1), 0, st, head);
-    if (vec)
-        hipLaunchKernelGGL((fmm_batch_kernel<TI, TL, TO, 0, true>), dim3(nwg), dim3(Cfg<0>::NT), 0, st, probs, items,
-                           nitems, head);
-    else
-        hipLaunchKernelGGL((fmm_batch_kernel<TI, TL, TO, 0, false>), dim3(nwg), dim3(Cfg<0>::NT), 0, st, probs, items,
-                           nitems, head);
+#define FMMK_BATCH(SH, V)                                                                                       \
+    hipLaunchKernelGGL((fmm_batch_kernel<TI, TL, TO, SH, V>), dim3(nwg), dim3(Cfg<SH>::NT), 0, st, probs, items, \
+                       nitems, head)
+    if (shape == 2) {
+        if (vec) FMMK_BATCH(2, true);
+        else FMMK_BATCH(2, false);
+    } else {
+        if (vec) FMMK_BATCH(0, true);
+        else FMMK_BATCH(0, false);
+    }
+#undef FMMK_BATCH
     return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
 }
 
@@ -648,8 +679,8 @@ int fmmk_launch_d_f32leaf(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn
                           hipStream_t st);
 int fmmk_launch_s32(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
 int fmmk_launch_batch_d64(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
-                          int nwg, hipStream_t st);
+                          int shape, int nwg, hipStream_t st);
 int fmmk_launch_batch_s32(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
-                          int nwg, hipStream_t st);
+                          int shape, int nwg, hipStream_t st);
 
 #endif

@@ -95,6 +95,7 @@ struct Req {
     long id = 0;
     double t_arr = 0.0, t_admit = 0.0, t_done = 0.0;
     int cand = -1;
+    int tile = 0;                   // 0: 128 x 128, 1: 64 x 64
     int status = -1;                // once done: 0, or the FMMHIP_ERR_* code
     bool done = false;
     size_t off1 = kNoBlock, bytes1 = 0;     // output + lists
@@ -136,7 +137,7 @@ struct fmmhip_server_s {
     void* dev = nullptr;
     Arena arena;
     std::vector<Slot> slots;
-    fmmhip_alg_t algs[kCands] = {nullptr, nullptr, nullptr};
+    fmmhip_alg_t algs[2][kCands] = {};          // [tile: 128 x 128, 64 x 64][candidate]
     size_t list_cap = 0;
     long next_id = 1;
     std::deque<long> waiting;                    // submitted, in arrival order
@@ -180,11 +181,14 @@ bool fmmhip_server_s::try_admit(Req& q, Slot& sl, int si)
 {
     const fmmhip_request_t& r = q.r;
     if (q.cand < 0) {
+        int edge = 128;
         if (r.cand >= 0) q.cand = r.cand;
-        else if (r.m > 0 && r.n > 0 && r.k > 0) fmmhip_tuner_choose(r.is_double, r.m, r.n, r.k, 1, &q.cand, nullptr);
+        else if (r.m > 0 && r.n > 0 && r.k > 0)
+            fmmhip_tuner_choose_tile(r.is_double, r.m, r.n, r.k, 1, &q.cand, &edge, nullptr);
         else q.cand = 0;
+        q.tile = edge == 64 ? 1 : 0;
     }
-    fmmhip_alg_t alg = algs[q.cand];
+    fmmhip_alg_t alg = algs[q.tile][q.cand];
     const size_t esz = r.is_double ? 8 : 4;
     const bool own_c = r.C == nullptr;
     const size_t c_bytes = own_c ? (esz * (size_t)std::max(r.m, 1L) * (size_t)std::max(r.n, 1L) + 255) & ~(size_t)255 : 0;
@@ -293,18 +297,25 @@ extern "C" int fmmhip_server_create(fmmhip_server_t* out, size_t hbm_budget, int
     if (rc) return rc;
     if (!out || slots < 1 || slots > 64 || hbm_budget < (1u << 20)) return FMMHIP_ERR_INVALID;
     fmmhip_server_t s = new fmmhip_server_s();
-    rc = fmmhip_alg_create_classical(&s->algs[0]);
-    if (!rc) rc = fmmhip_alg_create(&s->algs[1], &STRASSEN_FMM, 1);
     fmm_t two[2] = {STRASSEN_FMM, STRASSEN_FMM};
-    if (!rc) rc = fmmhip_alg_create(&s->algs[2], two, 2);
+    for (int t = 0; t < 2 && !rc; t++) {
+        rc = fmmhip_alg_create_classical(&s->algs[t][0]);
+        if (!rc) rc = fmmhip_alg_create(&s->algs[t][1], &STRASSEN_FMM, 1);
+        if (!rc) rc = fmmhip_alg_create(&s->algs[t][2], two, 2);
+        // shape 2: 64 x 64 tiles (taken where the 128 x 128 grid pads much)
+        for (int c = 0; c < kCands && !rc; c++) rc = fmmhip_alg_set_shape(s->algs[t][c], t ? 2 : -1);
+    }
     if (rc) { fmmhip_server_destroy(s); return rc; }
     // no product split: concurrent requests fill the device, and a split
     // would add up to 8 C-sized temporaries to every request's arena share;
     // the classical candidate keeps its k-split, taken only by requests too
     // small to occupy half the CUs (small temporaries; the choice depends
     // on the request alone, so a result is the same alone and in the mix)
-    for (int c = 1; c < kCands; c++) s->algs[c]->split = 1;
-    for (int c = 0; c < kCands; c++) s->list_cap = std::max(s->list_cap, (desc_bound(s->algs[c]) + 255) & ~(size_t)255);
+    for (int t = 0; t < 2; t++) {
+        for (int c = 1; c < kCands; c++) s->algs[t][c]->split = 1;
+        for (int c = 0; c < kCands; c++)
+            s->list_cap = std::max(s->list_cap, (desc_bound(s->algs[t][c]) + 255) & ~(size_t)255);
+    }
     if (hipMalloc(&s->dev, hbm_budget) != hipSuccess) {
         (void)hipGetLastError();
         s->dev = nullptr;
@@ -448,6 +459,7 @@ extern "C" void fmmhip_server_destroy(fmmhip_server_t s)
         if (sl.pinned) (void)hipHostFree(sl.pinned);
     }
     if (s->dev) (void)hipFree(s->dev);
-    for (int c = 0; c < kCands; c++) fmmhip_alg_destroy(s->algs[c]);
+    for (int t = 0; t < 2; t++)
+        for (int c = 0; c < kCands; c++) fmmhip_alg_destroy(s->algs[t][c]);
     delete s;
 }

@@ -153,6 +153,10 @@ _SIGS = {
     "fmmhip_batch_info": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.POINTER(_L),
                                ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_size_t)]),
     "fmmhip_batch_choice": (_I, [_P, _L, ctypes.POINTER(_I)]),
+    "fmmhip_batch_set_tile": (_I, [_P, _L, _I]),
+    "fmmhip_batch_tile": (_I, [_P, _L, ctypes.POINTER(_I)]),
+    "fmmhip_tuner_choose_tile": (_I, [_I, _L, _L, _L, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                      ctypes.POINTER(_D)]),
     "fmmhip_tuner_choose": (_I, [_I, _L, _L, _L, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "fmmhip_tuner_estimate": (_I, [_I, _I, _L, _L, _L, _I, ctypes.POINTER(_D)]),
     "fmmhip_tuner_calibrate": (_I, [_P]),
@@ -423,6 +427,16 @@ class Batch:
         check(lib().fmmhip_batch_choice(self.h, i, ctypes.byref(c)), "batch_choice")
         return c.value
 
+    def set_tile(self, i, tile):
+        """Force problem i's tile edge (64 or 128; 0 = the tuner's choice)."""
+        check(lib().fmmhip_batch_set_tile(self.h, i, tile), "batch_set_tile")
+
+    def tile(self, i):
+        """The planned tile edge of problem i."""
+        t = _I()
+        check(lib().fmmhip_batch_tile(self.h, i, ctypes.byref(t)), "batch_tile")
+        return t.value
+
     def close(self):
         if self.h:
             lib().fmmhip_batch_destroy(self.h)
@@ -514,6 +528,14 @@ def tuner_choose(is_double, m, n, k, lone=True):
     check(lib().fmmhip_tuner_choose(1 if is_double else 0, m, n, k, 1 if lone else 0, ctypes.byref(c),
                                     ctypes.byref(est)), "tuner_choose")
     return c.value, est.value
+
+
+def tuner_choose_tile(is_double, m, n, k, lone=True):
+    """(candidate, tile edge, estimated ms) the tuner picks for one problem."""
+    c, t, est = _I(), _I(), _D()
+    check(lib().fmmhip_tuner_choose_tile(1 if is_double else 0, m, n, k, 1 if lone else 0, ctypes.byref(c),
+                                         ctypes.byref(t), ctypes.byref(est)), "tuner_choose_tile")
+    return c.value, t.value, est.value
 
 
 def tuner_estimate(is_double, cand, m, n, k, lone=True):

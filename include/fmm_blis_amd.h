@@ -243,9 +243,13 @@ int  fmmhip_alg_info(fmmhip_alg_t alg, int* mt, int* kt, int* nt, int* R,
                      long* nnz_u, long* nnz_v, long* nnz_w);
 /* Number of equations of the Brent system the tables violate (0 = exact). */
 long fmmhip_alg_brent_violations(fmmhip_alg_t alg);
-/* Kernel launch shape for this algorithm: 0 = 512 threads (one part per
- * product loaded ahead), 1 = 256 threads (two parts ahead), -1 = chosen per
- * call (default).  Used by the autotuner. */
+/* Kernel launch shape for this algorithm: 0 = 128 x 128 tiles, 512 threads
+ * (one part per product loaded ahead), 1 = 128 x 128 tiles, 256 threads (two
+ * parts ahead), 2 = 64 x 64 tiles, 256 threads (column-major A, B and C
+ * without the classical k-split; other calls take shape 0), -1 = chosen per
+ * call (default: 0, or 1 for the fused pack mode).  The result does not
+ * depend on the shape (every element's sums run in the same order).  The
+ * batch's tuner chooses between shapes 0 and 2 per problem. */
 int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
 /* How the U/V-weighted operand sums are formed and the W-weighted products
  * applied: 0 = fused (operand sums streamed from the operands into LDS
@@ -396,22 +400,34 @@ int  fmmhip_batch_add_d(fmmhip_batch_t b, long m, long n, long k, double alpha, 
                         const double* B, long ldb, double beta, double* C, long ldc, int cand);
 int  fmmhip_batch_add_s(fmmhip_batch_t b, long m, long n, long k, float alpha, const float* A, long lda,
                         const float* B, long ldb, float beta, float* C, long ldc, int cand);
-/* Chooses every problem's candidate, prepares descriptors and the
- * workspace, and captures one HIP graph: per datatype one batched
- * summation launch and one persistent MFMA launch over (problem, tile)
- * items, costliest first, taken from a device queue. */
+/* Chooses every problem's candidate and tile (128 x 128 or 64 x 64: the
+ * smaller tile wastes less on padding where a leaf dimension is an odd
+ * multiple of 64, the larger runs faster on whole tiles), prepares
+ * descriptors and the workspace, and captures one HIP graph: per datatype
+ * one batched summation launch, then per tile one persistent MFMA launch
+ * over (problem, tile) items, costliest first, taken from a device queue.
+ * The tile does not change the result: each element of C is accumulated in
+ * the same order on either. */
 int  fmmhip_batch_plan(fmmhip_batch_t b);
 /* Launches the captured graph on `stream` (asynchronous; may be replayed). */
 int  fmmhip_batch_run(fmmhip_batch_t b, void* stream);
 int  fmmhip_batch_info(fmmhip_batch_t b, long* nprob, long* items_f64, long* items_f32, double* est_ms,
                        size_t* ws_bytes);
 int  fmmhip_batch_choice(fmmhip_batch_t b, long i, int* cand);
+/* Forces problem i's tile edge (64 or 128; 0 = the tuner's choice);
+ * invalidates a previous plan.  fmmhip_batch_tile reports the planned edge
+ * (the 128 x 128 kernel takes what the 64 x 64 one cannot). */
+int  fmmhip_batch_set_tile(fmmhip_batch_t b, long i, int tile);
+int  fmmhip_batch_tile(fmmhip_batch_t b, long i, int* tile);
 
 /* Tuner: the cost model's choice (and estimate, ms) for one problem, alone
  * on the device (lone = 1) or inside a batch; the estimate of a given
- * candidate; calibration by timing each candidate on the device; the
+ * candidate (on its faster tile); calibration by timing each candidate on the device; the
  * calibrated table saved to / loaded from a text file. */
 int fmmhip_tuner_choose(int is_double, long m, long n, long k, int lone, int* cand, double* est_ms);
+/* The same choice with its tile edge (64 or 128). */
+int fmmhip_tuner_choose_tile(int is_double, long m, long n, long k, int lone, int* cand, int* tile,
+                             double* est_ms);
 int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, long k, int lone, double* est_ms);
 int fmmhip_tuner_calibrate(void* stream);
 int fmmhip_tuner_save(const char* path);

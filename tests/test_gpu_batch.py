@@ -10,7 +10,11 @@ Bars (stated per test):
   * a problem's result in the batch equals the same problem run alone
     through the single-call entry with the same candidate, bit for bit;
   * every replay of the graph, with eager work between replays, reproduces
-    the first replay bit for bit.
+    the first replay bit for bit;
+  * the tile (128 x 128 or 64 x 64, chosen per problem or forced) does not
+    change any result: each element of C is accumulated in the same order on
+    either, so forced-64, forced-128 and tuner-chosen batches agree bit for
+    bit, and a single call on 64 x 64 tiles equals one on 128 x 128 tiles.
 """
 import sys
 
@@ -136,6 +140,77 @@ def test_batch_random_error_and_alone_equals_mix():
             F.sgemm_fmm(alg, m, n, k, p["alpha"], p["dA"], p["dB"], p["beta"], dC1)
         F.sync()
         assert np.array_equal(dC1.to_numpy(), got), (i, p["dt"], m, n, k, cand)
+
+
+def run_batch(integer, tile=None):
+    b, probs = build(integer)
+    if tile is not None:
+        for i in range(len(probs)):
+            b.set_tile(i, tile)
+    b.plan()
+    b.run()
+    F.sync()
+    outs = [p["dC"].to_numpy() for p in probs]
+    tiles = [b.tile(i) for i in range(len(probs))]
+    cands = [b.choice(i) for i in range(len(probs))]
+    b.close()
+    return outs, tiles, cands, probs
+
+
+def test_batch_tile_choice_does_not_change_results():
+    auto, t_auto, c_auto, probs = run_batch(False)
+    say("tiles chosen:", t_auto)
+    assert set(t_auto) == {64, 128}            # the mix holds problems for either tile
+    for tile in (64, 128):
+        outs, tiles, cands, _ = run_batch(False, tile)
+        say("forced", tile, tiles)
+        assert tiles == [tile] * len(MIX) and cands == c_auto
+        for i, (x, y) in enumerate(zip(auto, outs)):
+            assert np.array_equal(x, y), (tile, i, MIX[i])
+    # integers: exact on 64 x 64 tiles too
+    outs, tiles, cands, probs = run_batch(True, 64)
+    for i, p in enumerate(probs):
+        want = expected(p)
+        if p["dt"] == "f4":
+            want = want.astype(np.float32)
+        assert np.array_equal(outs[i], want), (i, MIX[i], cands[i])
+
+
+# (dtype, m, n, k, cand, pack): ragged edges, leaves that are odd multiples
+# of 64, every pack mode of the FP64 path
+SINGLE = [
+    ("f8", 320, 192, 448, 0, -1), ("f8", 1000, 700, 900, 1, 1), ("f8", 1000, 700, 900, 2, 1),
+    ("f8", 1000, 700, 900, 1, 0), ("f8", 1000, 700, 900, 2, 2), ("f8", 1, 5, 3, 0, -1),
+    ("f8", 513, 100, 77, 1, 1), ("f4", 320, 192, 448, 0, -1), ("f4", 1000, 700, 900, 2, 1),
+    ("f4", 97, 101, 103, 1, 0),
+]
+
+
+@pytest.mark.parametrize("case", SINGLE, ids=lambda c: "-".join(map(str, c)))
+def test_single_call_64_tiles_equal_128_tiles(case):
+    dt, m, n, k, cand, pack = case
+    i = SINGLE.index(case)
+    A, B, C = inputs(dt, m, n, k, 40 + i, False)
+    alpha, beta = ALPHA_BETA[i % len(ALPHA_BETA)]
+    dA, dB = dev(A), dev(B)
+    outs = []
+    for shape in (0, 2):
+        alg = alg_for(cand)
+        if cand > 0 and pack >= 0:
+            alg.set_pack(pack)
+        alg.set_shape(shape)
+        dC = dev(C)
+        fn = F.dgemm_fmm if dt == "f8" else F.sgemm_fmm
+        fn(alg, m, n, k, alpha, dA, dB, beta, dC)
+        F.sync()
+        outs.append(dC.to_numpy())
+        alg.close()
+    assert np.array_equal(outs[0], outs[1]), case
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    want = O.dgemm(alpha, np.asfortranarray(A64), np.asfortranarray(B64), beta,
+                   np.asfortranarray(C.astype(np.float64)))
+    err = O.normalized_error(outs[1].astype(np.float64), want, A64, B64)
+    assert err < (5e-15 if dt == "f8" else 1e-6), (case, err)
 
 
 def test_batch_graph_replays_with_eager_work_between():

@@ -241,12 +241,44 @@ def test_tuner_choices_follow_the_cost_model():
         F.tuner_choose(True, 0, 5, 5)
 
 
+def test_tuner_tile_choice_follows_padding():
+    """The tile the tuner picks (host logic only): 128 x 128 where the leaf
+    dimensions are multiples of 128 (the larger tile runs faster on whole
+    tiles), 64 x 64 where the 128 x 128 grid pads by more than the smaller
+    tile's rate loss (a leaf edge of 192 pads to 256, a third of the work
+    wasted), and 128 x 128 for a lone small classical problem (the
+    one-problem path splits its k instead)."""
+    assert F.tuner_choose_tile(True, 16384, 16384, 16384)[:2] == (2, 128)
+    assert F.tuner_choose_tile(True, 4096, 4096, 4096, lone=False)[1] == 128
+    assert F.tuner_choose_tile(True, 192, 192, 4096, lone=False)[:2] == (0, 64)
+    assert F.tuner_choose_tile(False, 576, 320, 2048, lone=False)[1] == 64
+    assert F.tuner_choose_tile(True, 192, 192, 4096, lone=True)[1] == 128
+    # 1280 = 10 x 128; its one-level leaf 640 = 5 x 128, its two-level leaf 320 = 5 x 64
+    c, t, est = F.tuner_choose_tile(True, 1280, 1280, 16384, lone=False)
+    assert t == (64 if c == 2 else 128)
+    # the reported estimate is the chosen candidate's on its faster tile
+    for (m, n, k) in [(192, 192, 4096), (4096, 4096, 4096), (3000, 1000, 700)]:
+        c, t, est = F.tuner_choose_tile(True, m, n, k, lone=False)
+        assert abs(F.tuner_estimate(True, c, m, n, k, lone=False) - est) < 1e-9
+        assert F.tuner_choose(True, m, n, k, lone=False) == (c, est)
+
+
 def test_tuner_table_save_load_round_trip(tmp_path):
     p = tmp_path / "tuner.txt"
     F.tuner_save(str(p))
     before = [F.tuner_estimate(True, c, 8192, 8192, 8192) for c in range(3)]
     text = p.read_text()
-    assert text.count("cand ") == 6 and "bw " in text
+    assert text.count("cand ") == 6 and "bw " in text and "tile_eff 1.000000 0.920000" in text
+    # the 64 x 64 tile's rate enters the tile choice: at half the rate its
+    # saved padding (256^2 against 192^2) no longer pays
+    assert F.tuner_choose_tile(True, 192, 192, 4096, lone=False)[1] == 64
+    slow = tmp_path / "slow64.txt"
+    slow.write_text(text.replace("tile_eff 1.000000 0.920000", "tile_eff 1.000000 0.500000"))
+    F.tuner_load(str(slow))
+    try:
+        assert F.tuner_choose_tile(True, 192, 192, 4096, lone=False)[1] == 128
+    finally:
+        F.tuner_load(str(p))
     # a table with another FP64 classical efficiency changes that estimate only
     lines = [ln if not ln.startswith("cand 0 f64") else "cand 0 f64 0.390000 4.000000" for ln in text.splitlines()]
     q = tmp_path / "t2.txt"
