@@ -135,8 +135,12 @@ typedef struct fmmhip_wsum_s {
      * first term, `part` = number of its terms */
     const fmm_entry_t* parts;
     /* device terms, per part in ascending product order: off = element
-     * offset of the product's slot, coef = W[c][r] */
+     * offset of the product's slot, coef = W[c][r], part = r */
     const fmm_entry_t* terms;
+    /* products the terms name (r < nslots), or 0: with np <= 16 and
+     * nslots <= 64 the stage forms every part in one pass that reads each
+     * slot once per element */
+    int nslots;
 } fmmhip_wsum_t;
 int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
 

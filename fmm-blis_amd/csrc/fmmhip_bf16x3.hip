@@ -703,6 +703,7 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
         w.np = mv.np;
         w.parts = mv.parts;
         w.terms = mv.terms;
+        w.nslots = R;
         return fmmhip_wsum(1, &w, st);
     }
     if (groups > 1) return fmmhip_split_reduce(1, C, m, n, rsC, csC, beta, ws + oT, tmp_elems, groups, st);

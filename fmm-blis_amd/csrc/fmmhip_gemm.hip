@@ -725,14 +725,17 @@ int split_groups(int requested, int max_groups, long tiles, double leaf_flops, d
 // Groups per tile of the products-materialised mode (no copies of C to
 // pay for): the G <= min(R, 64) that minimises the waves of workgroups
 // times the products of the longest group, ceil(tiles G / slots)
-// ceil(R / G); ties go to fewer groups.
+// ceil(R / G); ties go to more groups.  Blocks are group-major, so with
+// G = R the blocks of one product run together and share its S_r row and
+// T_r column panels in L2, as a classical GEMM's tiles share A and B (L2
+// hit rate 0.50 with one group, 0.77 with 49, at m = n = k = 16384, 222 L2).
 int mat_groups(int R, long tiles, int slots)
 {
     int best = 1;
     double best_t = 0.0;
     for (int g = 1; g <= std::min(R, 64); g++) {
         const double t = std::ceil((double)tiles * g / slots) * std::ceil((double)R / g);
-        if (g == 1 || t < best_t) { best = g; best_t = t; }
+        if (g == 1 || t <= best_t) { best = g; best_t = t; }
     }
     return best;
 }
@@ -745,8 +748,8 @@ int mat_groups(int R, long tiles, int slots)
 // materialised: mat_groups, no copies), plus, pre-summed, the fused W
 // epilogue's read-modify-write of one C leaf per W entry at its measured
 // effective rate (6.8 TB/s, partly hidden behind the MFMAs), or,
-// materialised, the slot writes and the W stage (one slot read per W
-// entry, C read and written once) at 4.9 TB/s.
+// materialised, the W stage (one slot read per W entry, C read and
+// written once) at 4.9 TB/s.
 bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, size_t esz, double c_span)
 {
     int cus = 256;
@@ -765,8 +768,14 @@ bool presum_wins(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int split, si
     const int g2 = split > 0 ? std::min(split, R) : mat_groups(R, tiles, slots);
     const double fill2 = ((double)tiles * R / slots) /
                          (std::ceil((double)tiles * g2 / slots) * std::ceil((double)R / g2));
+    // the slot writes hide behind the MFMAs of a long k loop: counted in
+    // the share min(1, 512 / lk), fitted to FP64 sweeps at 13824^3 and
+    // 13824^2 x 1152 (every algorithm, 1 and 2 levels) and to
+    // 16384^3 222 L2 (materialised kernel 99.2 ms, fused-epilogue 105.2 ms)
+    const double slot_share = std::min(1.0, 512.0 / (double)std::max(g.lk, 1L));
     const double t2 = leaf_flops / (rate * std::min(fill2, 1.0)) +
-                      ((double)R * leaf_c + (double)alg->nnz_w * leaf_c + 2.0 * (double)esz * c_span) / 4.9e12;
+                      (slot_share * (double)R * leaf_c + (double)alg->nnz_w * leaf_c + 2.0 * (double)esz * c_span) /
+                          4.9e12;
     return t1 <= t2;
 }
 
@@ -978,6 +987,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         w.np = alg->mt * alg->nt;
         w.parts = dv.mat_ent + 3 * alg->R;
         w.terms = w.parts + w.np;
+        w.nslots = alg->R;
         if (dv.wk.parts1) {
             // two-stage W: stage 1 into the Y leaves (after the slots), stage 2 into C
             P->w2 = true;
@@ -997,6 +1007,8 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
             w.np = dv.wk.np1;
             w.parts = dv.wk.parts1;
             w.terms = dv.wk.terms1;
+            w.nslots = 0;                        // the per-part pass for both stages
+            w2.nslots = 0;
         }
         P->nwg = tiles * ka.groups;
         if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
