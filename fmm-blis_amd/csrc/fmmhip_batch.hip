@@ -188,7 +188,9 @@ extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, lo
 // Large, device-filling launches give the steady-state efficiency that a
 // batch (no tails) and a busy server see; smaller calibration sizes folded
 // the launch tail into the classical figure and made the model favour FMM
-// where it measured slower.
+// where it measured slower.  The 64 x 64 tile's relative rate is the
+// classical FP64 kernel's time on 128 x 128 tiles over its time on
+// 64 x 64 tiles at the same size (65536 tiles: whole waves too).
 extern "C" int fmmhip_tuner_calibrate(void* stream)
 {
     int rc = fmmhip_init();
@@ -204,6 +206,7 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
     }
     tp.cus = cus;
     double bw_sum = 0.0, bw_n = 0.0;
+    double cls_ms = 0.0;              // classical FP64, 128 x 128 tiles, per launch
     for (int dbl = 0; dbl < 2 && !rc; dbl++) {
         const size_t esz = dbl ? 8 : 4;
         void *A = nullptr, *B = nullptr, *C = nullptr;
@@ -243,6 +246,7 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
             double kms = 0, pms = 0, mx = 0;
             fmmhip_timing_read(&nl, &kms, &pms, &mx);
             if (rc || nl <= 0) break;
+            if (dbl && cand == 0) cls_ms = kms / nl;
             const CandShape& c = kShape[cand];
             const double leaf = (double)nn / c.mt;
             const double fl = 2.0 * c.R * leaf * leaf * leaf;
@@ -256,6 +260,23 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
                 bw_sum += bytes / (pms / nl * 1e-3);
                 bw_n += 1;
             }
+        }
+        if (!rc && dbl && cls_ms > 0.0) {
+            fmmhip_alg_t a64 = nullptr;
+            rc = fmmhip_alg_create_classical(&a64);
+            if (!rc) rc = fmmhip_alg_set_shape(a64, 2);
+            if (!rc) rc = fmmhip_alg_set_split(a64, 1);
+            for (int rep = 0; rep < 3 && !rc; rep++) {
+                if (rep == 1) fmmhip_timing_enable(1);
+                rc = fmmhip_dgemm_fmm(a64, FMMHIP_LEAF_NATIVE, nmax, nmax, nmax, 1.0, (double*)A, 1, nmax,
+                                      (double*)B, 1, nmax, 1.0, (double*)C, 1, nmax, st);
+            }
+            fmmhip_timing_enable(0);
+            long nl = 0;
+            double kms = 0, pms = 0, mx = 0;
+            fmmhip_timing_read(&nl, &kms, &pms, &mx);
+            if (!rc && nl > 0 && kms > 0) tp.teff[1] = std::min(1.5, std::max(0.3, cls_ms / (kms / nl)));
+            fmmhip_alg_destroy(a64);
         }
         fmmhip_free(A); fmmhip_free(B); fmmhip_free(C);
     }

@@ -275,6 +275,9 @@ def test_tuner_calibration_on_device(tmp_path):
         say(text)
         effs = [float(l.split()[3]) for l in text.splitlines() if l.startswith("cand ")]
         assert len(effs) == 6 and all(0.2 < e <= 0.98 for e in effs)
+        # the 64 x 64 tile's measured rate relative to the 128 x 128 one
+        teff = [float(x) for l in text.splitlines() if l.startswith("tile_eff ") for x in l.split()[1:]]
+        assert teff[0] == 1.0 and 0.6 < teff[1] < 1.1, teff
         assert F.tuner_choose(True, 256, 256, 256)[0] == 0
         assert F.tuner_choose(True, 16384, 16384, 16384)[0] in (1, 2)
         F.tuner_load(str(after))
