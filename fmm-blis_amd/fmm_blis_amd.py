@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libfmm_blis_amd.so")
 # Datatype codes of obj_t.info bits 0-2 (blis/frame/include/bli_type_defs.h:378-381)
 BLIS_FLOAT = 0
 BLIS_DOUBLE = 2
+BLIS_CONSTANT = 5
 BLIS_SUCCESS = -1
 BLIS_FAILURE = -2
 TRANS_BIT = 1 << 3
@@ -61,6 +62,27 @@ ObjT._fields_ = [("root", ctypes.POINTER(ObjT)), ("off", ctypes.c_int64 * 2), ("
                  ("cs", ctypes.c_int64), ("is_", ctypes.c_int64), ("scalar", _Atom),
                  ("m_padded", ctypes.c_int64), ("n_padded", ctypes.c_int64), ("ps", ctypes.c_int64),
                  ("pd", ctypes.c_int64), ("m_panel", ctypes.c_int64), ("n_panel", ctypes.c_int64)]
+
+
+class ConstData(ctypes.Structure):
+    """Mirror of BLIS constdata_t (blis/frame/include/bli_type_defs.h:1200-1208),
+    the buffer of a BLIS_CONSTANT object such as BLIS_ONE: one value per
+    datatype, read by type (bli_obj_macro_defs.h:1286-1297)."""
+    _fields_ = [("s", ctypes.c_float), ("d", ctypes.c_double), ("c", ctypes.c_float * 2),
+                ("z", ctypes.c_double * 2), ("i", ctypes.c_int64)]
+
+
+def obj_constant(value_d, value_s=None):
+    """A BLIS_CONSTANT scalar object; returns (obj, storage) -- keep storage alive."""
+    cd = ConstData()
+    cd.s = value_d if value_s is None else value_s
+    cd.d = value_d
+    cd.c[0] = cd.s
+    cd.z[0] = value_d
+    cd.i = int(value_d)
+    o = ObjT()
+    lib().fmm_obj_attach(BLIS_CONSTANT, 1, 1, ctypes.cast(ctypes.pointer(cd), _P), 1, 1, ctypes.byref(o))
+    return o, cd
 
 
 class Request(ctypes.Structure):

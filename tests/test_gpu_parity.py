@@ -226,6 +226,29 @@ def test_dropin_ex_device_objects_and_trans(name):
     assert np.array_equal(dC.to_numpy(), want)
 
 
+@pytest.mark.parametrize("dt", ["f8", "f4"])
+def test_dropin_constant_scalars(dt):
+    """alpha and beta as BLIS_CONSTANT objects (the form of BLIS_ONE,
+    BLIS_MINUS_ONE), whose buffer holds one value per datatype: the double
+    value is taken (here 2 and -1; the float fields hold 3 and 5 so a wrong
+    offset shows), for double and float operands alike."""
+    m, n, k = 130, 100, 90
+    A, B, C = O.integers(m, k, 9, 0, 3), O.integers(k, n, 9, 1, 3), O.integers(m, n, 9, 2, 3)
+    want = O.dgemm(2.0, A, B, -1.0, C.copy(order="F"))
+    code = F.BLIS_DOUBLE if dt == "f8" else F.BLIS_FLOAT
+    if dt == "f4":
+        A, B, C = (np.asfortranarray(X.astype(np.float32)) for X in (A, B, C))
+        want = want.astype(np.float32)
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    oa = F.obj_attach(dA.ptr.value, m, k, 1, m, dt=code)
+    ob = F.obj_attach(dB.ptr.value, k, n, 1, k, dt=code)
+    oc = F.obj_attach(dC.ptr.value, m, n, 1, m, dt=code)
+    al, s1 = F.obj_constant(2.0, value_s=3.0)
+    be, s2 = F.obj_constant(-1.0, value_s=5.0)
+    F.lib().bli_strassen_ab_ex(al, oa, ob, be, oc, fmm("222"))
+    assert np.array_equal(dC.to_numpy(), want)
+
+
 def test_dropin_row_stored_host_and_levels():
     m, n, k = 120, 100, 90
     A, B, C = O.integers(m, k, 8, 0, 3), O.integers(k, n, 8, 1, 3), O.integers(m, n, 8, 2, 3)

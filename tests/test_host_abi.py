@@ -217,6 +217,16 @@ def test_c_header_layout_matches_binding(tmp_path):
     assert got == want
 
 
+def test_constdata_layout_matches_the_library_read():
+    """BLIS_CONSTANT scalars (BLIS_ONE, ...) carry a constdata_t
+    (bli_type_defs.h:1200-1208: float s, double d, scomplex c, dcomplex z,
+    gint_t i); the library reads the double value at byte 8, which is where
+    the platform C ABI puts d after a 4-byte float."""
+    C = F.ConstData
+    assert (C.s.offset, C.d.offset, C.c.offset, C.z.offset, C.i.offset) == (0, 8, 16, 24, 40)
+    assert ctypes.sizeof(C) == 48
+
+
 def test_tuner_choices_follow_the_cost_model():
     """The tuner's choices (host logic only): classical for small or thin
     problems, two-level Strassen for FP64 16384^3; a level needs leaves of
