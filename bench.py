@@ -570,6 +570,30 @@ def main():
         cls_gflops = 2.0 * m * n * k / (cls_ms * 1e-3) / 1e9
         Cc.free()
 
+    # the other pack modes of an FMM call, for the record beside the chosen
+    # one: operand sums formed while staging (fused, no workspace) and the W
+    # scatter fused into the MFMA kernel (pre-summed, no product in HBM)
+    modes = None
+    if args.alg != "classical" and args.pack == -1 and args.leaf == "f64" and args.classical_steps > 0:
+        modes = {}
+        Cm = F.DeviceMatrix(m, n).fill_uniform(0, 2)
+        for pk, name in ((0, "fused"), (1, "pre-summed"), (2, "pre-summed, products materialised")):
+            a_m = F.Alg([f] * args.levels)
+            a_m.set_pack(pk)
+            F.dgemm_fmm(a_m, m, n, k, 1.0, A, B, 1.0, Cm)
+            F.sync()
+            me = [(F.Event(), F.Event()) for _ in range(2)]
+            for e0, e1 in me:
+                e0.record()
+                F.dgemm_fmm(a_m, m, n, k, 1.0, A, B, 1.0, Cm)
+                e1.record()
+            F.sync()
+            mms = sum(e0.elapsed_ms(e1) for e0, e1 in me) / len(me)
+            modes[name] = {"ms": round(mms, 3), "effective_gflops": round(2.0 * m * n * k / (mms * 1e-3) / 1e9, 1)}
+            a_m.close()
+            F.check(F.lib().fmmhip_workspace_release(), "workspace_release")
+        Cm.free()
+
     # accuracy: one fresh FMM and one fresh classical product from the same C
     rel = None
     if not args.no_check:
@@ -642,6 +666,7 @@ def main():
             "rel_err_vs_classical": rel,
             "summation_ms_per_step": round(presum_ms, 3),     # materialised: the W stage included
             "precision_form": bf3,
+            "pack_modes": modes,
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3),
                        "mean": round(sum(lat) / len(lat), 3)},
