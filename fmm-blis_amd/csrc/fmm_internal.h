@@ -135,12 +135,17 @@ typedef struct fmmhip_wsum_s {
      * first term, `part` = number of its terms */
     const fmm_entry_t* parts;
     /* device terms, per part in ascending product order: off = element
-     * offset of the product's slot, coef = W[c][r], part = r */
+     * offset of the product's slot, coef = W[c][r], part = r, cols = the
+     * slot's index inside the part's group (below) */
     const fmm_entry_t* terms;
-    /* products the terms name (r < nslots), or 0: with np <= 16 and
-     * nslots <= 64 the stage forms every part in one pass that reads each
-     * slot once per element */
+    /* All-parts pass (nslots > 0): the parts fall into ngroups groups of at
+     * most 16 parts whose terms name at most nslots <= 64 slots each; a
+     * group's parts are formed together, reading each of its slots once per
+     * element.  groups: ngroups x gparts part indices (-1: none), or NULL
+     * for one group of all np parts.  nslots = 0: one pass per part. */
     int nslots;
+    int ngroups, gparts;
+    const int32_t* groups;
 } fmmhip_wsum_t;
 int fmmhip_wsum(int is_double, const fmmhip_wsum_t* w, void* stream);
 

@@ -704,6 +704,8 @@ int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alph
         w.parts = mv.parts;
         w.terms = mv.terms;
         w.nslots = R;
+        w.ngroups = 1;
+        w.gparts = mv.np;
         return fmmhip_wsum(1, &w, st);
     }
     if (groups > 1) return fmmhip_split_reduce(1, C, m, n, rsC, csC, beta, ws + oT, tmp_elems, groups, st);

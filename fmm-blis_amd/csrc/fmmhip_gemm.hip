@@ -255,6 +255,7 @@ int build_mat_list(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, const De
             t.off = (int64_t)r * sl.slot;
             t.coef = wf[e].coef;
             t.part = r;
+            t.cols = r;          // slot index in the single all-parts group
         }
     return FMMHIP_SUCCESS;
 }
@@ -489,6 +490,7 @@ int build_wkron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, DevDesc* d,
                 t.off = ((long)r0 * R1 + r1) * slot;
                 t.coef = (double)w;
                 t.part = r0 * R1 + r1;
+                t.cols = r1;                     // slot index in group r0
                 t1.push_back(t);
             }
             pe.part = (int32_t)t1.size() - pe.first;
@@ -508,11 +510,22 @@ int build_wkron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, DevDesc* d,
                         t.off = ((long)r0 * np1 + c1) * slot;
                         t.coef = (double)w;
                         t.part = r0;
+                        t.cols = r0;             // slot index in group c1
                         t2.push_back(t);
                     }
                     pe.part = (int32_t)t2.size() - pe.first;
                 }
-    (void)np0;
+    // all-parts groups: stage 1 group r0 = Y[r0][0..np1) over slots
+    // r0*R1 .. r0*R1+R1-1; stage 2 group c1 = C parts (c0, c1) over Y[0..R0)[c1]
+    std::vector<int32_t> g1((size_t)R0 * np1), g2((size_t)np1 * np0);
+    for (int r0 = 0; r0 < R0; r0++)
+        for (int c1 = 0; c1 < np1; c1++) g1[(size_t)r0 * np1 + c1] = r0 * np1 + c1;
+    for (int a0 = 0; a0 < rows0; a0++)
+        for (int b0 = 0; b0 < cols0; b0++)
+            for (int a1 = 0; a1 < rows1; a1++)
+                for (int b1 = 0; b1 < cols1; b1++)
+                    g2[(size_t)(a1 * cols1 + b1) * np0 + (a0 * cols0 + b0)] =
+                        (a0 * rows1 + a1) * nt + (b0 * cols1 + b1);
     std::vector<char>& h = d->wk_host;
     h.clear();
     auto place = [&](const void* src, size_t bytes) {
@@ -525,6 +538,8 @@ int build_wkron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, DevDesc* d,
     const size_t o2 = place(t1.data(), t1.size() * sizeof(fmm_entry_t));
     const size_t o3 = place(p2.data(), p2.size() * sizeof(fmm_entry_t));
     const size_t o4 = place(t2.data(), t2.size() * sizeof(fmm_entry_t));
+    const size_t o5 = place(g1.data(), g1.size() * sizeof(int32_t));
+    const size_t o6 = place(g2.data(), g2.size() * sizeof(int32_t));
     if (hipMalloc(&d->wk_dev, h.size()) != hipSuccess) {
         (void)hipGetLastError();
         d->wk_dev = nullptr;
@@ -539,6 +554,10 @@ int build_wkron(const struct fmmhip_alg_s* alg, const fmm_geom_t& g, DevDesc* d,
     v.np1 = R0 * np1;
     v.np2 = npc;
     v.y_leaves = (long)R0 * np1;
+    v.groups1 = (const int32_t*)(dev + o5);
+    v.groups2 = (const int32_t*)(dev + o6);
+    v.ngroups1 = R0; v.gparts1 = np1; v.nslots1 = R1;
+    v.ngroups2 = np1; v.gparts2 = np0; v.nslots2 = R0;
     if (hipMemcpyAsync(d->wk_dev, h.data(), h.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
         return FMMHIP_ERR_HIP;
     return FMMHIP_SUCCESS;
@@ -988,6 +1007,9 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         w.parts = dv.mat_ent + 3 * alg->R;
         w.terms = w.parts + w.np;
         w.nslots = alg->R;
+        w.ngroups = 1;
+        w.gparts = w.np;
+        w.groups = nullptr;
         if (dv.wk.parts1) {
             // two-stage W: stage 1 into the Y leaves (after the slots), stage 2 into C
             P->w2 = true;
@@ -1007,8 +1029,14 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
             w.np = dv.wk.np1;
             w.parts = dv.wk.parts1;
             w.terms = dv.wk.terms1;
-            w.nslots = 0;                        // the per-part pass for both stages
-            w2.nslots = 0;
+            w.nslots = dv.wk.nslots1;            // all-parts groups for both stages
+            w.ngroups = dv.wk.ngroups1;
+            w.gparts = dv.wk.gparts1;
+            w.groups = dv.wk.groups1;
+            w2.nslots = dv.wk.nslots2;
+            w2.ngroups = dv.wk.ngroups2;
+            w2.gparts = dv.wk.gparts2;
+            w2.groups = dv.wk.groups2;
         }
         P->nwg = tiles * ka.groups;
         if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;

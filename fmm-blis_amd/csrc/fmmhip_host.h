@@ -30,6 +30,10 @@ struct WKronView {
     const fmm_entry_t *parts2, *terms2;   // C parts and their terms over the Y leaves
     int np1, np2;
     long y_leaves;
+    // all-parts groups: stage 1 one per outer product (its inner C parts),
+    // stage 2 one per inner C part (the outer C parts that hold it)
+    const int32_t *groups1, *groups2;
+    int ngroups1, gparts1, nslots1, ngroups2, gparts2, nslots2;
 };
 
 struct Prepared {

@@ -15,12 +15,14 @@
 // consecutive threads walk c, so the slot reads and the C accesses coalesce.
 //
 // Two passes: per part (wsum_kernel: each part reads its products, so a
-// product is read once per W entry), or, for at most 16 parts and 64
-// products, all parts at once (wsum_cols_kernel: a thread keeps every
-// part's value at its element in registers and reads each product once,
-// applying it to the parts that take it; 222 L2: 49 slot reads and 32 C
-// accesses per element instead of 144 and 32).  Both run the same FMA chain
-// per element, so they give the same C.
+// product is read once per W entry), or all parts of a group at once
+// (wsum_cols_kernel, groups of at most 16 parts over at most 64 slots: a
+// thread keeps every part's value at its element in registers and reads
+// each slot once, applying it to the parts that take it; 222 L2, one
+// group: 49 slot reads and 32 C accesses per element instead of 144 and
+// 32; the two stages of 444 L2: 49 groups of 16 Y leaves over 49 products,
+// then 16 groups of 16 C parts over 49 Y leaves).  Both run the same FMA
+// chain per element, so they give the same C.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -104,10 +106,11 @@ __global__ __launch_bounds__(kThreads) void wsum_kernel(fmmhip_wsum_t w, int cbl
 constexpr int kColsMaxR = 64;    // products of the all-parts pass
 constexpr int kColsMaxP = 16;    // parts of the all-parts pass
 
-// All parts at once.  Block item bb: strided index s, chunk cb of
-// kThreads*kEpt elements along c; grid-stride over the items.  The block
-// first lays out the W table by product in LDS: lam[r][p] = alpha W[p][r]
-// and bit p of mask[r] set where part p takes product r.
+// All parts of one group at once.  blockIdx.y: the group; block item bb:
+// strided index s, chunk cb of kThreads*kEpt elements along c; grid-stride
+// over the items.  The block first lays out its group's W table by slot in
+// LDS: lam[j][p] = alpha W (part p, slot j), bit p of mask[j] set where
+// part p takes slot j, and the slot offsets.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void wsum_cols_kernel(fmmhip_wsum_t w, int cblocks, long nblocks)
 {
@@ -115,18 +118,28 @@ __global__ __launch_bounds__(kThreads) void wsum_cols_kernel(fmmhip_wsum_t w, in
     __shared__ long off_s[kColsMaxR];
     __shared__ unsigned char has_s[kColsMaxR][kColsMaxP];
     __shared__ unsigned mask_s[kColsMaxR];
+    __shared__ int part_s[kColsMaxP];
     const T* __restrict__ M = (const T*)w.M;
     T* __restrict__ C = (T*)w.C;
     const int tid = threadIdx.x;
-    const int np = w.np, R = w.nslots;
+    const int R = w.nslots;
+    const int g = (int)blockIdx.y;
+    const int np = w.groups ? w.gparts : w.np;      // parts of this group (<= kColsMaxP, host-checked)
+    if (tid < kColsMaxP) {
+        int pi = -1;
+        if (tid < np) pi = w.groups ? w.groups[(long)g * w.gparts + tid] : tid;
+        part_s[tid] = (pi >= 0 && pi < w.np) ? pi : -1;
+    }
     for (int i = tid; i < kColsMaxR * kColsMaxP; i += kThreads) (&has_s[0][0])[i] = 0;
     for (int i = tid; i < kColsMaxR; i += kThreads) off_s[i] = 0;
     __syncthreads();
     for (int p = 0; p < np; p++) {
-        const fmm_entry_t pe = w.parts[p];
+        const int pi = part_s[p];
+        if (pi < 0) continue;                // uniform
+        const fmm_entry_t pe = w.parts[pi];
         for (int j = tid; j < pe.part; j += kThreads) {
             const fmm_entry_t te = w.terms[pe.first + j];
-            const int r = te.part;
+            const int r = te.cols;               // the slot's index in the group
             if (r >= 0 && r < R) {
                 lam_s[r][p] = w.alpha * te.coef;
                 has_s[r][p] = 1;
@@ -157,8 +170,8 @@ __global__ __launch_bounds__(kThreads) void wsum_cols_kernel(fmmhip_wsum_t w, in
         for (int p = 0; p < kColsMaxP; p++) {
 #pragma unroll
             for (int u = 0; u < kEpt; u++) acc[p][u] = 0.0;
-            if (p < np && w.beta != 0.0) {
-                const fmm_entry_t pe = w.parts[p];
+            if (p < np && part_s[p] >= 0 && w.beta != 0.0) {
+                const fmm_entry_t pe = w.parts[part_s[p]];
                 const int ec = w.swap ? pe.cols : pe.rows, es = w.swap ? pe.rows : pe.cols;
                 if (s < es) {
                     const T* cp = C + pe.off + (long)s * cs_s;
@@ -194,8 +207,8 @@ __global__ __launch_bounds__(kThreads) void wsum_cols_kernel(fmmhip_wsum_t w, in
         }
 #pragma unroll
         for (int p = 0; p < kColsMaxP; p++) {
-            if (p < np) {
-                const fmm_entry_t pe = w.parts[p];
+            if (p < np && part_s[p] >= 0) {
+                const fmm_entry_t pe = w.parts[part_s[p]];
                 const int ec = w.swap ? pe.cols : pe.rows, es = w.swap ? pe.rows : pe.cols;
                 if (s < es) {
                     T* cp = C + pe.off + (long)s * cs_s;
@@ -213,12 +226,16 @@ int launch(const fmmhip_wsum_t* w, hipStream_t st)
 {
     if (w->np <= 0 || w->Lc <= 0 || w->Ls <= 0) return FMMHIP_SUCCESS;
     const int cblocks = (w->Lc + kThreads * kEpt - 1) / (kThreads * kEpt);
-    if (w->nslots > 0 && w->nslots <= kColsMaxR && w->np <= kColsMaxP) {
+    const bool grouped = w->groups && w->ngroups > 0 && w->gparts > 0 && w->gparts <= kColsMaxP;
+    const bool single = !w->groups && w->np <= kColsMaxP;
+    if (w->nslots > 0 && w->nslots <= kColsMaxR && (grouped || single)) {
         const long nblocks = (long)w->Ls * cblocks;
+        const int ng = grouped ? w->ngroups : 1;
         int cus = 256;
         fmmhip_device_cu_count(&cus);
-        const long grid = std::min(nblocks, 8L * cus);
-        hipLaunchKernelGGL(wsum_cols_kernel<T>, dim3((unsigned)grid), dim3(kThreads), 0, st, *w, cblocks, nblocks);
+        const long per = std::max(1L, std::min(nblocks, 8L * cus / ng));   // blocks per group
+        hipLaunchKernelGGL(wsum_cols_kernel<T>, dim3((unsigned)per, (unsigned)ng), dim3(kThreads), 0, st, *w,
+                           cblocks, nblocks);
         return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
     }
     const long nblocks = (long)w->np * w->Ls * cblocks;
