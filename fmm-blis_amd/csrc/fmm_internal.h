@@ -41,6 +41,7 @@ struct fmmhip_alg_s {
     int  shape;                 /* kernel launch shape, -1 = chosen per call */
     int  pack;                  /* 0 fused, 1 pre-summed workspace, -1 = chosen per call */
     int  last_pack;             /* mode the last FP64/FP32 call took (-1: none yet) */
+    int  last_shape;            /* kernel shape the last FP64/FP32 call launched (-1: none yet) */
     int  keep_all;              /* never evict cached descriptors (owners that capture graphs) */
     int  split;                 /* product groups per tile: -1 chosen per call, 1 (none), 2, 4, 8 */
     /* two-level algorithms: the two levels' own tables (level 0 outer),

@@ -317,6 +317,7 @@ int fmmhip_alg_create(fmmhip_alg_t* out, const fmm_t* algs, int nlevels)
     g->split = -1;
     g->pack = -1;
     g->last_pack = -1;
+    g->last_shape = -1;
     /* start from level 0 and fold the next levels in as inner blocks */
     int cm = algs[0].m_tilde, ck = algs[0].k_tilde, cn = algs[0].n_tilde, cR = algs[0].R;
     int* U = (int*)malloc(sizeof(int) * (size_t)(cm * ck * cR));

@@ -822,6 +822,17 @@ int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int
     return presum_wins(alg, kind, g, split, esz, c_span) ? 1 : 2;
 }
 
+// The fast FP64 kernel (shape 3) for calls that may take it: single-part
+// operands whose prepared tiles all lie inside their parts (automatic shape,
+// or shape 3 asked for); otherwise the prepared shape stays.
+void pick_fast(Prepared* P, const struct fmmhip_alg_s* alg, Kind kind, bool single_part)
+{
+    const bool want = alg->shape < 0 || alg->shape == 3;
+    const bool ok = kind == Kind::D64 && single_part && fmmk::fast_eligible(P->ka, P->a_mn, P->b_mn, P->c_col);
+    if ((P->shape == 0 || P->shape == 3) && want && ok) P->shape = 3;
+    else if (P->shape == 3) P->shape = 0;
+}
+
 int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alpha, const void* A, long rsA,
                  long csA, const void* B, long rsB, long csB, double beta, void* C, long rsC, long csC, int pack,
                  int split, hipStream_t st, Prepared* P, DescSource* src)
@@ -952,6 +963,9 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     }
     // the 64 x 64 tile shape is instantiated for column-major A, B, C only
     if (shape == 2 && !(a_mn && !b_mn && rsC == 1)) shape = 0;
+    // shape 3, the fast FP64 kernel, takes single-part operands only
+    const bool single_part = presum || !multi;
+    if (shape == 3 && !(kind == Kind::D64 && single_part)) shape = 0;
     P->shape = shape;
     P->a_mn = a_mn;
     P->b_mn = b_mn;
@@ -1041,6 +1055,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         P->nwg = tiles * ka.groups;
         if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
         if (tiles <= 0) P->nothing = true;
+        pick_fast(P, alg, kind, single_part);
         return FMMHIP_SUCCESS;
     }
     ka.groups = split_groups(split, alg->R == 1 ? (kspl ? (int)std::min(ksteps / 8, 8L) : 1) : alg->R, tiles,
@@ -1060,6 +1075,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     P->nwg = tiles * ka.groups;
     if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
     if (tiles <= 0) P->nothing = true;
+    pick_fast(P, alg, kind, single_part);
     return FMMHIP_SUCCESS;
 }
 
@@ -1218,7 +1234,10 @@ static int gemm_common(fmmhip_alg_t alg, Kind kind, long m, long n, long k, doub
         }
     }
     rc = launch_prepared(&P);
-    if (!rc && !P.nothing && !P.scale_only) alg->last_pack = P.mode;
+    if (!rc && !P.nothing && !P.scale_only) {
+        alg->last_pack = P.mode;
+        alg->last_shape = P.shape;
+    }
     return rc;
 }
 
@@ -1385,7 +1404,7 @@ fmmhip_alg_t classical_alg()
 
 extern "C" int fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape)
 {
-    if (!alg || shape < -1 || shape > 2) return FMMHIP_ERR_INVALID;
+    if (!alg || shape < -1 || shape > 3) return FMMHIP_ERR_INVALID;
     alg->shape = shape;
     return FMMHIP_SUCCESS;
 }
@@ -1438,6 +1457,14 @@ extern "C" int fmmhip_alg_last_pack(fmmhip_alg_t alg, int* pack)
     if (!alg || !pack) return FMMHIP_ERR_INVALID;
     std::unique_lock<std::mutex> lk = enqueue_lock();
     *pack = alg->last_pack;
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_alg_last_shape(fmmhip_alg_t alg, int* shape)
+{
+    if (!alg || !shape) return FMMHIP_ERR_INVALID;
+    std::unique_lock<std::mutex> lk = enqueue_lock();
+    *shape = alg->last_shape;
     return FMMHIP_SUCCESS;
 }
 

@@ -568,6 +568,173 @@ void fmm_gemm_kernel(KArgs a)
     gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL, KSPL>(a, tm, tn, grp, sA, sB);
 }
 
+// Fast FP64 kernel for single-part operands (classical, pre-summed,
+// products materialised) whose tiles all lie inside their parts:
+// column-major A, k-contiguous B, column-major C, leaf extents multiples of
+// the 128 x 128 tile and of BK, 16-byte aligned pairs, no k-split.  256
+// threads as 2 x 2 waves of 64 x 64, two workgroups per CU.  Per k-step the
+// next tile is loaded through pointers walked per step, the fragments of
+// sub-step ks+1 are read into the other register set while the MFMAs of
+// sub-step ks run (the reads spread evenly among the MFMAs), and the next
+// tile is stored at the end.  The MFMAs of each accumulator run in the
+// same order as in gemm_tile, and the W epilogue is its unclipped form, so
+// results equal the other shapes' bit for bit.
+constexpr int kFastNT = 256;
+struct FastCfg { static constexpr int T = 128, WAVES = 2, WT = 64, FI = 4, FJ = 4; };
+
+template <int UNUSED>   // a template, so each translation unit may instantiate it
+__global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
+void fmm_fast_kernel(KArgs a)
+{
+    constexpr int T = FastCfg::T, FI = FastCfg::FI, FJ = FastCfg::FJ, KS = BK / 4;
+    constexpr int RS = lds_row<double, T>();
+    __shared__ double sA[2][lds_tile<double, T>()];
+    __shared__ double sB[2][lds_tile<double, T>()];
+    // staging geometry: pairs along m for A (k rows), along k for B (n columns)
+    constexpr int AP = T * BK / 2 / kFastNT, A_MNP = T / 2, A_KR = kFastNT / A_MNP;
+    constexpr int B_KP = BK / 2, B_NC = kFastNT / B_KP;
+    static_assert(AP * A_KR == BK && AP * B_NC == T, "staging covers the tile");
+    typedef Leaf<double>::acc_t acc_t;
+
+    const int tiles = a.tiles_m * a.tiles_n;
+    const int grp = (int)blockIdx.x / tiles;
+    int tm, tn;
+    tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave % FastCfg::WAVES) * FastCfg::WT, wn = (wave / FastCfg::WAVES) * FastCfg::WT;
+    const int m0 = tm * T, n0 = tn * T;
+    const int a_m = 2 * (tid % A_MNP), a_k = tid / A_MNP;
+    const int b_k = 2 * (tid % B_KP), b_n = tid / B_KP;
+    const long lda = a.lda, ldb = a.ldb;
+    const bool split = a.groups > 1;
+    double* __restrict__ Cb = split ? (double*)a.tmp + (long)grp * a.tmp_elems : (double*)a.C;
+    const int R = a.R;
+    const int r_begin = (int)((long)grp * R / a.groups), r_end = (int)((long)(grp + 1) * R / a.groups);
+    const int* __restrict__ uptr = a.ptr;
+    const int* __restrict__ vptr = a.ptr + (R + 1);
+    const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
+    const fmm_entry_t* __restrict__ ent = a.ent;
+    const int nk = a.lk / BK;
+    const int l16 = lane & 15, kq = lane >> 4;
+
+    d2_t ra[AP], rb[AP];
+    auto lstore = [&](int buf) {
+        double* la = sA[buf];
+        double* lb = sB[buf];
+#pragma unroll
+        for (int s = 0; s < AP; s++) {
+            double* q = la + (a_k + A_KR * s) * RS + a_m;
+            q[0] = ra[s][0];
+            q[1] = ra[s][1];
+        }
+#pragma unroll
+        for (int s = 0; s < AP; s++) {
+            double* q = lb + b_k * RS + b_n + B_NC * s;
+            q[0] = rb[s][0];
+            q[RS] = rb[s][1];
+        }
+    };
+    double fa[2][FI], fb[2][FJ];
+    auto frag = [&](int buf, int ks, int f) {
+        const double* la = sA[buf];
+        const double* lb = sB[buf];
+        const int kk = 4 * ks + kq;
+#pragma unroll
+        for (int i = 0; i < FI; i++) fa[f][i] = la[kk * RS + wm + 16 * i + l16];
+#pragma unroll
+        for (int j = 0; j < FJ; j++) fb[f][j] = lb[kk * RS + wn + 16 * j + l16];
+    };
+
+    for (int r = r_begin; r < r_end; r++) {
+        const fmm_entry_t ea = ent[uptr[r]], eb = ent[vptr[r]];
+        const double* pa = (const double*)a.A + ea.off + m0 + a_m + (long)a_k * lda;
+        const double* pb = (const double*)a.B + eb.off + (long)(n0 + b_n) * ldb + b_k;
+        const long da = (long)BK * lda;
+        auto gload = [&]() {
+#pragma unroll
+            for (int s = 0; s < AP; s++) ra[s] = *(const d2_t*)(pa + (long)(A_KR * s) * lda);
+#pragma unroll
+            for (int s = 0; s < AP; s++) rb[s] = *(const d2_t*)(pb + (long)(B_NC * s) * ldb);
+            pa += da;
+            pb += BK;
+        };
+        auto scale = [&]() {
+            if (ea.coef != 1.0) {
+#pragma unroll
+                for (int s = 0; s < AP; s++) ra[s] *= ea.coef;
+            }
+            if (eb.coef != 1.0) {
+#pragma unroll
+                for (int s = 0; s < AP; s++) rb[s] *= eb.coef;
+            }
+        };
+        acc_t acc[FI][FJ];
+#pragma unroll
+        for (int i = 0; i < FI; i++)
+#pragma unroll
+            for (int j = 0; j < FJ; j++) acc[i][j] = (acc_t){0, 0, 0, 0};
+        gload();
+        scale();
+        lstore(0);
+        __syncthreads();
+        frag(0, 0, 0);
+        for (int kt = 0; kt < nk; kt++) {
+            const int cur = kt & 1;
+            const bool more = kt + 1 < nk;
+            if (more) gload();
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                const int f = ks & 1;
+                if (ks + 1 < KS) frag(cur, ks + 1, f ^ 1);
+#pragma unroll
+                for (int i = 0; i < FI; i++)
+#pragma unroll
+                    for (int j = 0; j < FJ; j++) acc[i][j] = Leaf<double>::mfma(fa[f][i], fb[f][j], acc[i][j]);
+                if (ks + 1 < KS) {
+                    constexpr int NR = (FI + FJ) / 2, NM = FI * FJ;
+#pragma unroll
+                    for (int q = 0; q < NR; q++) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, NM / NR, 0);   // MFMA
+                        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);         // LDS read
+                    }
+                }
+                if (ks == KS - 1 && more) {
+                    scale();
+                    lstore(cur ^ 1);
+                }
+            }
+            __syncthreads();
+            if (more) frag(cur ^ 1, 0, 0);
+        }
+        // W epilogue, every tile inside every C part (unclipped form)
+        const int er0 = wm + Leaf<double>::ROW_LANE * kq;
+        const int ec0 = wn + l16;
+        for (int e = wptr[r]; e < wptr[r + 1]; e++) {
+            const fmm_entry_t we = ent[e];
+            const double lam = a.alpha * we.coef;
+            const bool first = ((we.first >> a.gbit) & 1) != 0;
+            const double beta = split ? 0.0 : a.beta;
+            const int cs32 = (int)a.csC;
+            double* ct = Cb + we.off + (long)(m0 + er0) + (long)(n0 + ec0) * a.csC;
+            const bool rd = !first || beta != 0.0;
+#pragma unroll
+            for (int j = 0; j < FJ; j++)
+                epi_block<false, FI, Leaf<double>::ROW_REG, true>(ct + 16 * j * cs32, 1, 0, 0, acc, j, lam, first, beta,
+                                                                  rd);
+        }
+    }
+}
+
+// Whether a prepared FP64 launch can take fmm_fast_kernel (single-part
+// operands are the caller's to guarantee).
+inline bool fast_eligible(const KArgs& ka, bool a_mn, bool b_mn, bool c_col)
+{
+    constexpr int T = FastCfg::T;
+    return a_mn && !b_mn && c_col && ka.va && ka.vb && !ka.ksplit && ka.lm % T == 0 && ka.ln % T == 0 &&
+           ka.lk % BK == 0 && ka.lk >= BK && ka.min_m == ka.lm && ka.min_n == ka.ln && ka.min_k == ka.lk &&
+           ka.min_cm >= ka.lm && ka.min_cn >= ka.ln && ka.tiles_m * T == ka.lm && ka.tiles_n * T == ka.ln;
+}
+
 // Batched persistent kernel: a fixed grid of workgroups takes work items
 // (problem, tile) from a device queue until it is empty.  Items are listed
 // most expensive first; the queue head is one counter, reset by
@@ -654,6 +821,16 @@ int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, i
     if (shape == 1) {
         if (vec) return launch_layouts<TI, TL, TO, 1, true>(ka, a_mn, b_mn, c_col, nwg, st);
         return launch_layouts<TI, TL, TO, 1, false>(ka, a_mn, b_mn, c_col, nwg, st);
+    }
+    if (shape == 3) {
+        // the fast FP64 kernel (the host selects it for eligible single-part calls)
+        if constexpr (sizeof(TI) == 8 && sizeof(TL) == 8 && sizeof(TO) == 8) {
+            if (!fast_eligible(ka, a_mn, b_mn, c_col)) return FMMHIP_ERR_UNSUPPORTED;
+            hipLaunchKernelGGL(fmm_fast_kernel<0>, dim3(nwg), dim3(kFastNT), 0, st, ka);
+            return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
+        } else {
+            return FMMHIP_ERR_UNSUPPORTED;
+        }
     }
     if (vec) return launch_layouts<TI, TL, TO, 0, true>(ka, a_mn, b_mn, c_col, nwg, st);
     return launch_layouts<TI, TL, TO, 0, false>(ka, a_mn, b_mn, c_col, nwg, st);

@@ -145,6 +145,7 @@ _SIGS = {
     "fmmhip_alg_set_pack": (_I, [_P, _I]),
     "fmmhip_alg_pack_for": (_I, [_P, _I, _L, _L, _L, ctypes.POINTER(_I)]),
     "fmmhip_alg_last_pack": (_I, [_P, ctypes.POINTER(_I)]),
+    "fmmhip_alg_last_shape": (_I, [_P, ctypes.POINTER(_I)]),
     "fmmhip_alg_set_split": (_I, [_P, _I]),
     "fmmhip_workspace_release": (_I, []),
     "fmmhip_timing_enable": (_I, [_I]),
@@ -292,6 +293,12 @@ class Alg:
         """Pack mode the last FP64/FP32-leaf call took (-1 before the first)."""
         p = ctypes.c_int(-1)
         check(lib().fmmhip_alg_last_pack(self.h, ctypes.byref(p)), "alg_last_pack")
+        return p.value
+
+    def last_shape(self):
+        """Kernel shape the last FP64/FP32 call launched (-1 before the first)."""
+        p = ctypes.c_int(-1)
+        check(lib().fmmhip_alg_last_shape(self.h, ctypes.byref(p)), "alg_last_shape")
         return p.value
 
     def set_split(self, groups):

@@ -213,6 +213,44 @@ def test_single_call_64_tiles_equal_128_tiles(case):
     assert err < (5e-15 if dt == "f8" else 1e-6), (case, err)
 
 
+# (m, n, k, cand, pack): FP64 shapes whose leaves the fast kernel takes
+FAST = [(1024, 1024, 1024, 0, -1), (2048, 1024, 1536, 1, 1), (2048, 2048, 2048, 2, 1), (2048, 2048, 2048, 2, 2),
+        (1024, 1024, 512, 0, -1)]
+
+
+@pytest.mark.parametrize("case", FAST, ids=lambda c: "-".join(map(str, c)))
+def test_fast_kernel_equals_shape0(case):
+    """The automatic shape takes the fast FP64 kernel (shape 3) on these
+    shapes, and its C equals the 512-thread kernel's (shape 0) bit for bit,
+    with the product split of the materialised mode too; integer inputs
+    equal the oracle."""
+    m, n, k, cand, pack = case
+    i = FAST.index(case)
+    for integer in (False, True):
+        A, B, C = inputs("f8", m, n, k, 60 + i, integer)
+        alpha, beta = (0.7, -1.3) if not integer else (2.0, -1.0)
+        dA, dB = dev(A), dev(B)
+        outs, shapes = [], []
+        for shape in (-1, 0):
+            alg = alg_for(cand)
+            if cand > 0:
+                alg.set_pack(pack)
+            alg.set_shape(shape)
+            if pack == 2:
+                alg.set_split(-1)
+            dC = dev(C)
+            F.dgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC)
+            F.sync()
+            outs.append(dC.to_numpy())
+            shapes.append(alg.last_shape())
+            alg.close()
+        assert shapes == [3, 0], shapes
+        assert np.array_equal(outs[0], outs[1]), (case, integer)
+        if integer:
+            want = O.dgemm(alpha, np.asfortranarray(A), np.asfortranarray(B), beta, np.asfortranarray(C.copy()))
+            assert np.array_equal(outs[0], want), case
+
+
 def test_batch_graph_replays_with_eager_work_between():
     b, probs = build(integer=False)
     b.plan()
