@@ -672,7 +672,8 @@ def main():
                        "mean": round(sum(lat) / len(lat), 3)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": read_traffic(tag),
-                     "kernel": "bf16x3_gemm_kernel" if args.leaf.startswith("bf16") else "fmm_gemm_kernel",
+                     "kernel": ("bf16x3_gemm_kernel" if args.leaf.startswith("bf16") else
+                                "fmm_fast_kernel" if alg.last_shape() == 3 else "fmm_gemm_kernel"),
                      "kernel_ms": round(kern_ms, 3),
                      "mfma_flops_per_launch": mfma_fl, "effective_over_peak": round(eff / 1e3 / peak, 4)},
     }
