@@ -344,10 +344,11 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
             const bool more = rn < r_end;
             if (more) BF3_GLOAD(rn, kbn);
 
-            // the plane products of the step; LLVM's MFMA/LDS-read interleaving
-            // strategy 0 (3% faster than none at 16384^3, classical and 222 L2;
-            // strategy 1 and 2 measured 0-4% apart from it)
-            __builtin_amdgcn_iglp_opt(0);
+            // the plane products of the step; with four planes LLVM's
+            // MFMA/LDS-read interleaving strategy 0 (3% faster than none at
+            // 16384^3, classical and 222 L2; strategies 1 and 2 measured 0-4%
+            // apart from it); the three-plane forms run 2.6x slower with it
+            if (PLANES == 4) __builtin_amdgcn_iglp_opt(0);
             const u32x4* la = sA[st];
             const u32x4* lb = sB[st];
             auto fa = [&](int g, int i) { return __builtin_bit_cast(bf16x8, la[rd_a + (g * BM + 16 * i) * 4]); };
