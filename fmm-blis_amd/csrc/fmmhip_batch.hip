@@ -407,7 +407,8 @@ struct fmmhip_batch_s {
             if (!rc) rc = fmmhip_alg_create(&algs[t][2], two, 2);
             for (int c = 0; c < kCands && !rc; c++) {
                 algs[t][c]->keep_all = 1;
-                rc = fmmhip_alg_set_shape(algs[t][c], kTileShape[t]);
+                // 128 x 128 tiles: the fast FP64 kernel where a problem allows it (shape 3; others 0)
+                rc = fmmhip_alg_set_shape(algs[t][c], kTileShape[t] == 0 ? 3 : kTileShape[t]);
             }
         }
         return rc;
@@ -516,13 +517,17 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         for (size_t i = 0; i < np; i++)
             if (b->prep[i].presum) bind_workspace(&b->prep[i], (char*)b->ws + ws_off[i]);
     }
-    // 2. per datatype and tile (group g = 2 * tile + datatype): kernel
-    //    arguments and work items (costliest first); per datatype: summation
-    //    segments and their block offsets
-    constexpr int kGroups = 2 * kTiles;
+    // 2. per datatype and launch shape (group g = 2 * s + datatype, s = 0
+    //    for 128 x 128 tiles, 1 for 64 x 64, 2 for the fast FP64 kernel):
+    //    kernel arguments and work items (costliest first); per datatype:
+    //    summation segments and their block offsets
+    constexpr int kShapes = 3;
+    const int kShapeOf[kShapes] = {0, 2, 3};
+    constexpr int kGroups = 2 * kShapes;
     std::vector<KArgs> ka[kGroups];
     std::vector<BatchItem> items[kGroups];
-    std::vector<double> cost[kGroups];
+    std::vector<BatchItem> qitems[kGroups][fmmk::kBatchQueues];
+    std::vector<double> qcost[kGroups][fmmk::kBatchQueues];
     std::vector<fmmhip_presum_seg_t> segs[2];
     std::vector<int> boff[2];
     bool vec[kGroups];
@@ -535,14 +540,24 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         const int d = b->probs[i].dbl;
         if (P.nothing) continue;
         if (P.scale_only) { scale_only.push_back(i); continue; }
-        const int g = 2 * (P.shape == 2 ? 1 : 0) + d;
+        const int g = 2 * (P.shape == 2 ? 1 : P.shape == 3 ? 2 : 0) + d;
         const int pi = (int)ka[g].size();
         ka[g].push_back(P.ka);
         vec[g] = vec[g] && P.ka.va && P.ka.vb;
         const double c = (double)P.ka.R * (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
-        for (long t = 0; t < P.nwg; t++) {
-            items[g].push_back(BatchItem{pi, (int)t});
-            cost[g].push_back(c);
+        // the problem's tiles in blocks of 8 tile rows (column-major inside a
+        // block), dealt to the queues in contiguous runs
+        const int tiles_m = P.ka.tiles_m, tiles_n = P.ka.tiles_n;
+        const long ntiles = (long)tiles_m * tiles_n;
+        for (long lin = 0; lin < ntiles; lin++) {
+            const long per_group = 8L * tiles_n;
+            const long gi = lin / per_group, first_m = gi * 8;
+            const long gsz = std::min<long>(tiles_m - first_m, 8);
+            const long in = lin - gi * per_group;
+            const long tm = first_m + in % gsz, tn = in / gsz;
+            const int q = (int)(lin * fmmk::kBatchQueues / ntiles);
+            qitems[g][q].push_back(BatchItem{pi, (int)(tm + tn * tiles_m)});
+            qcost[g][q].push_back(c);
         }
         if (P.presum) {
             for (const fmmhip_presum_seg_t* sg : {&P.segA, &P.segB}) {
@@ -551,14 +566,19 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             }
         }
     }
+    // each queue costliest first, the queues of a group laid end to end
     b->items[0] = b->items[1] = 0;
+    std::vector<int> qoff[kGroups];
     for (int g = 0; g < kGroups; g++) {
-        std::vector<size_t> order(items[g].size());
-        for (size_t j = 0; j < order.size(); j++) order[j] = j;
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[g][x] > cost[g][y]; });
-        std::vector<BatchItem> sorted(order.size());
-        for (size_t j = 0; j < order.size(); j++) sorted[j] = items[g][order[j]];
-        items[g].swap(sorted);
+        qoff[g].push_back(0);
+        for (int q = 0; q < fmmk::kBatchQueues; q++) {
+            std::vector<size_t> order(qitems[g][q].size());
+            for (size_t j = 0; j < order.size(); j++) order[j] = j;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t x, size_t y) { return qcost[g][q][x] > qcost[g][q][y]; });
+            for (size_t j = 0; j < order.size(); j++) items[g].push_back(qitems[g][q][order[j]]);
+            qoff[g].push_back((int)items[g].size());
+        }
         b->items[g % 2] += (long)items[g].size();
     }
     // block ranges of the summation segments: part-staged units are 256
@@ -572,12 +592,13 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     // 3. one device block: [ka per group | items per group | segs0 | segs1 | boff0 | boff1 | heads]
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) & ~(size_t)255; return o; };
-    size_t o_ka[kGroups], o_it[kGroups], o_sg[2], o_bo[2];
+    size_t o_ka[kGroups], o_it[kGroups], o_qo[kGroups], o_sg[2], o_bo[2];
     for (int g = 0; g < kGroups; g++) o_ka[g] = take(sizeof(KArgs) * (ka[g].size() + 1));
     for (int g = 0; g < kGroups; g++) o_it[g] = take(sizeof(BatchItem) * (items[g].size() + 1));
+    for (int g = 0; g < kGroups; g++) o_qo[g] = take(sizeof(int) * (fmmk::kBatchQueues + 1));
     for (int d = 0; d < 2; d++) o_sg[d] = take(sizeof(fmmhip_presum_seg_t) * (segs[d].size() + 1));
     for (int d = 0; d < 2; d++) o_bo[d] = take(sizeof(int) * (boff[d].size() + 1));
-    const size_t o_head = take(kGroups * sizeof(int));
+    const size_t o_head = take(kGroups * fmmk::kBatchQueues * sizeof(int));
     if (hipMalloc(&b->dev, off) != hipSuccess) { (void)hipGetLastError(); b->dev = nullptr; return FMMHIP_ERR_NOMEM; }
     char* dv = (char*)b->dev;
     for (int g = 0; g < kGroups; g++) {
@@ -586,6 +607,9 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             return FMMHIP_ERR_HIP;
         if (!items[g].empty() && hipMemcpy(dv + o_it[g], items[g].data(), sizeof(BatchItem) * items[g].size(),
                                            hipMemcpyHostToDevice) != hipSuccess)
+            return FMMHIP_ERR_HIP;
+        if (hipMemcpy(dv + o_qo[g], qoff[g].data(), sizeof(int) * qoff[g].size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
             return FMMHIP_ERR_HIP;
     }
     for (int d = 0; d < 2; d++) {
@@ -610,17 +634,19 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         if (!segs[d].empty())
             rc = fmmhip_presum_batch(d, (const fmmhip_presum_seg_t*)(dv + o_sg[d]), (int)segs[d].size(),
                                      (const int*)(dv + o_bo[d]), boff[d].back(), parts[d] ? 1 : 0, b->stream);
-        for (int t = 0; t < kTiles && !rc; t++) {
+        for (int t = 0; t < kShapes && !rc; t++) {
             const int g = 2 * t + d;
             if (items[g].empty()) continue;
-            const int nwg = (int)std::min<long>((long)items[g].size(), (long)kTileWgPerCu[t] * cus);
-            int* head = (int*)(dv + o_head) + g;
+            const int shape = kShapeOf[t];
+            const int nwg = (int)std::min<long>((long)items[g].size(), (shape == 2 ? 4L : 2L) * cus);
+            int* heads = (int*)(dv + o_head) + g * fmmk::kBatchQueues;
+            const int* qo = (const int*)(dv + o_qo[g]);
             if (d)
-                rc = fmmk_launch_batch_d64((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]),
-                                           (int)items[g].size(), head, vec[g], kTileShape[t], nwg, b->stream);
+                rc = fmmk_launch_batch_d64((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]), qo, heads,
+                                           vec[g], shape, nwg, b->stream);
             else
-                rc = fmmk_launch_batch_s32((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]),
-                                           (int)items[g].size(), head, vec[g], kTileShape[t], nwg, b->stream);
+                rc = fmmk_launch_batch_s32((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]), qo, heads,
+                                           vec[g], shape, nwg, b->stream);
         }
     }
     hipGraph_t g = nullptr;

@@ -7,8 +7,8 @@ int fmmk_launch_s32(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool
     return fmmk::launch_typed<float, float, float>(ka, shape, a_mn, b_mn, c_col, nwg, st);
 }
 
-int fmmk_launch_batch_s32(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
-                          int shape, int nwg, hipStream_t st)
+int fmmk_launch_batch_s32(const fmmk::KArgs* probs, const fmmk::BatchItem* items, const int* qoff, int* heads,
+                          bool vec, int shape, int nwg, hipStream_t st)
 {
-    return fmmk::launch_batch_typed<float, float, float>(probs, items, nitems, head, vec, shape, nwg, st);
+    return fmmk::launch_batch_typed<float, float, float>(probs, items, qoff, heads, vec, shape, nwg, st);
 }

@@ -582,24 +582,20 @@ void fmm_gemm_kernel(KArgs a)
 constexpr int kFastNT = 256;
 struct FastCfg { static constexpr int T = 128, WAVES = 2, WT = 64, FI = 4, FJ = 4; };
 
-template <int UNUSED>   // a template, so each translation unit may instantiate it
-__global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
-void fmm_fast_kernel(KArgs a)
+// One tile position (tm, tn) of product group grp: the body of the fast
+// kernel and of its batched persistent form.
+__device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int grp,
+                                          double (*sA)[lds_tile<double, FastCfg::T>()],
+                                          double (*sB)[lds_tile<double, FastCfg::T>()])
 {
     constexpr int T = FastCfg::T, FI = FastCfg::FI, FJ = FastCfg::FJ, KS = BK / 4;
     constexpr int RS = lds_row<double, T>();
-    __shared__ double sA[2][lds_tile<double, T>()];
-    __shared__ double sB[2][lds_tile<double, T>()];
     // staging geometry: pairs along m for A (k rows), along k for B (n columns)
     constexpr int AP = T * BK / 2 / kFastNT, A_MNP = T / 2, A_KR = kFastNT / A_MNP;
     constexpr int B_KP = BK / 2, B_NC = kFastNT / B_KP;
     static_assert(AP * A_KR == BK && AP * B_NC == T, "staging covers the tile");
     typedef Leaf<double>::acc_t acc_t;
 
-    const int tiles = a.tiles_m * a.tiles_n;
-    const int grp = (int)blockIdx.x / tiles;
-    int tm, tn;
-    tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % FastCfg::WAVES) * FastCfg::WT, wn = (wave / FastCfg::WAVES) * FastCfg::WT;
     const int m0 = tm * T, n0 = tn * T;
@@ -725,6 +721,20 @@ void fmm_fast_kernel(KArgs a)
     }
 }
 
+template <int UNUSED>   // a template, so each translation unit may instantiate it
+__global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
+void fmm_fast_kernel(KArgs a)
+{
+    __shared__ double sA[2][lds_tile<double, FastCfg::T>()];
+    __shared__ double sB[2][lds_tile<double, FastCfg::T>()];
+    // blocks [g*tiles, (g+1)*tiles) are product group g
+    const int tiles = a.tiles_m * a.tiles_n;
+    const int grp = (int)blockIdx.x / tiles;
+    int tm, tn;
+    tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
+    fast_tile(a, tm, tn, grp, sA, sB);
+}
+
 // Whether a prepared FP64 launch can take fmm_fast_kernel (single-part
 // operands are the caller's to guarantee).
 inline bool fast_eligible(const KArgs& ka, bool a_mn, bool b_mn, bool c_col)
@@ -736,31 +746,51 @@ inline bool fast_eligible(const KArgs& ka, bool a_mn, bool b_mn, bool c_col)
 }
 
 // Batched persistent kernel: a fixed grid of workgroups takes work items
-// (problem, tile) from a device queue until it is empty.  Items are listed
-// most expensive first; the queue head is one counter, reset by
-// batch_reset_kernel before each launch.
+// (problem, tile) from device queues until they are empty.  There is one
+// queue per XCD (workgroup b of a grid that is resident at once runs on XCD
+// b mod 8): the host deals each problem's tiles to the queues in contiguous
+// blocks of its tile grid, so the workgroups of one XCD share A and B panels
+// in that XCD's L2, and lists each queue most expensive first.  A workgroup
+// takes from its own queue, then from the next ones once it is empty.  The
+// queue heads are counters, reset by batch_reset_kernel before each launch.
+constexpr int kBatchQueues = 8;
 struct BatchItem {
     int problem;
     int tile;                    // linear tile index in the problem's leaf grid
 };
 
+// The next work item of the workgroup (uniform), or -1 when every queue
+// from `q` on is empty; q and tries carry the workgroup's position.
+__device__ __forceinline__ int batch_next(const int* __restrict__ qoff, int* __restrict__ heads, int& q, int& tries,
+                                          int* slot)
+{
+    for (;;) {
+        if (threadIdx.x == 0) *slot = atomicAdd(&heads[q], 1);
+        __syncthreads();
+        // uniform across the workgroup: readfirstlane keeps the item and its
+        // problem descriptor in scalar registers
+        const int w = __builtin_amdgcn_readfirstlane(*slot);
+        __syncthreads();                       // everyone has read slot before it is rewritten
+        const int q0 = qoff[q], qn = qoff[q + 1] - q0;
+        if (w < qn) return q0 + w;
+        if (++tries == kBatchQueues) return -1;
+        q = (q + 1) % kBatchQueues;
+    }
+}
+
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC>
 __global__ __attribute__((amdgpu_flat_work_group_size(Cfg<SHAPE>::NT, Cfg<SHAPE>::NT),
                           amdgpu_waves_per_eu(Cfg<SHAPE>::OCC, Cfg<SHAPE>::OCC)))
-void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items, int nitems,
-                      int* __restrict__ head)
+void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items,
+                      const int* __restrict__ qoff, int* __restrict__ heads)
 {
     __shared__ TL sA[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
     __shared__ TL sB[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
     __shared__ int slot;
+    int q = (int)blockIdx.x % kBatchQueues, tries = 0;
     for (;;) {
-        if (threadIdx.x == 0) slot = atomicAdd(head, 1);
-        __syncthreads();
-        // uniform across the workgroup: readfirstlane keeps the item and its
-        // problem descriptor in scalar registers
-        const int w = __builtin_amdgcn_readfirstlane(slot);
-        __syncthreads();                       // everyone has read slot before it is rewritten
-        if (w >= nitems) break;
+        const int w = batch_next(qoff, heads, q, tries, &slot);
+        if (w < 0) break;
         const BatchItem it = items[w];
         const KArgs a = probs[it.problem];
         const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
@@ -769,8 +799,33 @@ void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restri
     }
 }
 
+// The batched form of the fast kernel (problems the host found eligible;
+// one product group per tile).
+template <int UNUSED>
+__global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
+void fmm_batch_fast_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items,
+                           const int* __restrict__ qoff, int* __restrict__ heads)
+{
+    __shared__ double sA[2][lds_tile<double, FastCfg::T>()];
+    __shared__ double sB[2][lds_tile<double, FastCfg::T>()];
+    __shared__ int slot;
+    int q = (int)blockIdx.x % kBatchQueues, tries = 0;
+    for (;;) {
+        const int w = batch_next(qoff, heads, q, tries, &slot);
+        if (w < 0) break;
+        const BatchItem it = items[w];
+        const KArgs a = probs[it.problem];
+        const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
+        fast_tile(a, tm, tn, 0, sA, sB);
+        __syncthreads();                       // LDS free before the next item's prologue
+    }
+}
+
 template <int UNUSED>   // a template, so each translation unit may instantiate it
-__global__ void batch_reset_kernel(int* head) { *head = 0; }
+__global__ void batch_reset_kernel(int* heads)
+{
+    if (threadIdx.x < kBatchQueues) heads[threadIdx.x] = 0;
+}
 
 // Turns runtime layout flags into template arguments.
 template <typename TI, typename TL, typename TO, int SHAPE, bool VEC>
@@ -837,19 +892,24 @@ int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, i
 }
 
 // Batched launch (column-major A, B, C): reset the queue head, then the
-// persistent kernel of launch shape `shape` (0: 128 x 128 tiles, 2: 64 x 64)
+// persistent kernel of launch shape `shape` (0, 3: 128 x 128 tiles, 2: 64 x 64)
 // on `nwg` workgroups.
 template <typename TI, typename TL, typename TO>
-int launch_batch_typed(const KArgs* probs, const BatchItem* items, int nitems, int* head, bool vec, int shape, int nwg,
-                       hipStream_t st)
+int launch_batch_typed(const KArgs* probs, const BatchItem* items, const int* qoff, int* heads, bool vec, int shape,
+                       int nwg, hipStream_t st)
 {
-    hipLaunchKernelGGL(batch_reset_kernel<0>, dim3(1), dim3(1), 0, st, head);
+    hipLaunchKernelGGL(batch_reset_kernel<0>, dim3(1), dim3(64), 0, st, heads);
 #define FMMK_BATCH(SH, V)                                                                                       \
     hipLaunchKernelGGL((fmm_batch_kernel<TI, TL, TO, SH, V>), dim3(nwg), dim3(Cfg<SH>::NT), 0, st, probs, items, \
-                       nitems, head)
+                       qoff, heads)
     if (shape == 2) {
         if (vec) FMMK_BATCH(2, true);
         else FMMK_BATCH(2, false);
+    } else if (shape == 3) {
+        if constexpr (sizeof(TI) == 8 && sizeof(TL) == 8 && sizeof(TO) == 8)
+            hipLaunchKernelGGL(fmm_batch_fast_kernel<0>, dim3(nwg), dim3(kFastNT), 0, st, probs, items, qoff, heads);
+        else
+            return FMMHIP_ERR_UNSUPPORTED;
     } else {
         if (vec) FMMK_BATCH(0, true);
         else FMMK_BATCH(0, false);
@@ -865,9 +925,10 @@ int fmmk_launch_d64(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool
 int fmmk_launch_d_f32leaf(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg,
                           hipStream_t st);
 int fmmk_launch_s32(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
-int fmmk_launch_batch_d64(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
-                          int shape, int nwg, hipStream_t st);
-int fmmk_launch_batch_s32(const fmmk::KArgs* probs, const fmmk::BatchItem* items, int nitems, int* head, bool vec,
-                          int shape, int nwg, hipStream_t st);
+// items: kBatchQueues queues, queue q at items[qoff[q] .. qoff[q+1]); heads: kBatchQueues counters
+int fmmk_launch_batch_d64(const fmmk::KArgs* probs, const fmmk::BatchItem* items, const int* qoff, int* heads,
+                          bool vec, int shape, int nwg, hipStream_t st);
+int fmmk_launch_batch_s32(const fmmk::KArgs* probs, const fmmk::BatchItem* items, const int* qoff, int* heads,
+                          bool vec, int shape, int nwg, hipStream_t st);
 
 #endif
