@@ -822,13 +822,14 @@ int resolve_pack(fmmhip_alg_t alg, Kind kind, const fmm_geom_t& g, int pack, int
     return presum_wins(alg, kind, g, split, esz, c_span) ? 1 : 2;
 }
 
-// The fast FP64 kernel (shape 3) for calls that may take it: single-part
+// The fast kernel (shape 3) for calls that may take it: single-part
 // operands whose prepared tiles all lie inside their parts (automatic shape,
 // or shape 3 asked for); otherwise the prepared shape stays.
 void pick_fast(Prepared* P, const struct fmmhip_alg_s* alg, Kind kind, bool single_part)
 {
     const bool want = alg->shape < 0 || alg->shape == 3;
-    const bool ok = kind == Kind::D64 && single_part && fmmk::fast_eligible(P->ka, P->a_mn, P->b_mn, P->c_col);
+    const bool ok = single_part && (kind == Kind::D64 ? fmmk::fast_eligible<double>(P->ka, P->a_mn, P->b_mn, P->c_col)
+                                                      : fmmk::fast_eligible<float>(P->ka, P->a_mn, P->b_mn, P->c_col));
     if ((P->shape == 0 || P->shape == 3) && want && ok) P->shape = 3;
     else if (P->shape == 3) P->shape = 0;
 }
@@ -963,9 +964,9 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     }
     // the 64 x 64 tile shape is instantiated for column-major A, B, C only
     if (shape == 2 && !(a_mn && !b_mn && rsC == 1)) shape = 0;
-    // shape 3, the fast FP64 kernel, takes single-part operands only
+    // shape 3, the fast kernel, takes single-part operands only
     const bool single_part = presum || !multi;
-    if (shape == 3 && !(kind == Kind::D64 && single_part)) shape = 0;
+    if (shape == 3 && !single_part) shape = 0;
     P->shape = shape;
     P->a_mn = a_mn;
     P->b_mn = b_mn;

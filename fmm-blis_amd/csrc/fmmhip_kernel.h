@@ -568,33 +568,41 @@ void fmm_gemm_kernel(KArgs a)
     gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL, KSPL>(a, tm, tn, grp, sA, sB);
 }
 
-// Fast FP64 kernel for single-part operands (classical, pre-summed,
-// products materialised) whose tiles all lie inside their parts:
-// column-major A, k-contiguous B, column-major C, leaf extents multiples of
-// the 128 x 128 tile and of BK, 16-byte aligned pairs, no k-split.  256
-// threads as 2 x 2 waves of 64 x 64, two workgroups per CU.  Per k-step the
-// next tile is loaded through pointers walked per step, the fragments of
-// sub-step ks+1 are read into the other register set while the MFMAs of
-// sub-step ks run (the reads spread evenly among the MFMAs), and the next
-// tile is stored at the end.  The MFMAs of each accumulator run in the
-// same order as in gemm_tile, and the W epilogue is its unclipped form, so
-// results equal the other shapes' bit for bit.
+// Fast kernel for single-part operands (classical, pre-summed, products
+// materialised) whose tiles all lie inside their parts: column-major A,
+// k-contiguous B, column-major C, leaf extents multiples of the 128 x 128
+// tile and of the k-step, 16-byte aligned pairs, no k-split.  256 threads as
+// 2 x 2 waves of 64 x 64, two workgroups per CU; k-steps of 16 (FP64 leaf)
+// or 32 (FP32 leaf: its MFMAs take half the time, so twice the k per
+// barrier).  Per k-step the next tile is loaded through pointers walked per
+// step, the fragments of sub-step ks+1 are read into the other register set
+// while the MFMAs of sub-step ks run (the reads spread evenly among the
+// MFMAs), and the next tile is stored at the end.  The MFMAs of each
+// accumulator run in the same k order as in gemm_tile, operands are scaled
+// and rounded to the leaf type the same way, and the W epilogue is its
+// unclipped form, so results equal the other shapes' bit for bit.
 constexpr int kFastNT = 256;
 struct FastCfg { static constexpr int T = 128, WAVES = 2, WT = 64, FI = 4, FJ = 4; };
+template <typename TL>
+__device__ __host__ constexpr int fast_bk() { return sizeof(TL) == 8 ? 16 : 32; }
+template <typename TL>
+__device__ __host__ constexpr int fast_lds() { return fast_bk<TL>() * lds_row<TL, FastCfg::T>(); }
 
 // One tile position (tm, tn) of product group grp: the body of the fast
 // kernel and of its batched persistent form.
-__device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int grp,
-                                          double (*sA)[lds_tile<double, FastCfg::T>()],
-                                          double (*sB)[lds_tile<double, FastCfg::T>()])
+template <typename TI, typename TL, typename TO>
+__device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int grp, TL (*sA)[fast_lds<TL>()],
+                                          TL (*sB)[fast_lds<TL>()])
 {
-    constexpr int T = FastCfg::T, FI = FastCfg::FI, FJ = FastCfg::FJ, KS = BK / 4;
-    constexpr int RS = lds_row<double, T>();
+    constexpr int T = FastCfg::T, FI = FastCfg::FI, FJ = FastCfg::FJ, FBK = fast_bk<TL>(), KS = FBK / 4;
+    constexpr int RS = lds_row<TL, T>();
     // staging geometry: pairs along m for A (k rows), along k for B (n columns)
-    constexpr int AP = T * BK / 2 / kFastNT, A_MNP = T / 2, A_KR = kFastNT / A_MNP;
-    constexpr int B_KP = BK / 2, B_NC = kFastNT / B_KP;
-    static_assert(AP * A_KR == BK && AP * B_NC == T, "staging covers the tile");
-    typedef Leaf<double>::acc_t acc_t;
+    constexpr int AP = T * FBK / 2 / kFastNT, A_MNP = T / 2, A_KR = kFastNT / A_MNP;
+    constexpr int B_KP = FBK / 2, B_NC = kFastNT / B_KP;
+    static_assert(AP * A_KR == FBK && AP * B_NC == T, "staging covers the tile");
+    static_assert(KS % 2 == 0, "the fragment register sets alternate per sub-step");
+    typedef typename Leaf<TL>::acc_t acc_t;
+    typedef typename Pair<TI>::t pair_t;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % FastCfg::WAVES) * FastCfg::WT, wn = (wave / FastCfg::WAVES) * FastCfg::WT;
@@ -603,37 +611,21 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
     const int b_k = 2 * (tid % B_KP), b_n = tid / B_KP;
     const long lda = a.lda, ldb = a.ldb;
     const bool split = a.groups > 1;
-    double* __restrict__ Cb = split ? (double*)a.tmp + (long)grp * a.tmp_elems : (double*)a.C;
+    TO* __restrict__ Cb = split ? (TO*)a.tmp + (long)grp * a.tmp_elems : (TO*)a.C;
     const int R = a.R;
     const int r_begin = (int)((long)grp * R / a.groups), r_end = (int)((long)(grp + 1) * R / a.groups);
     const int* __restrict__ uptr = a.ptr;
     const int* __restrict__ vptr = a.ptr + (R + 1);
     const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
     const fmm_entry_t* __restrict__ ent = a.ent;
-    const int nk = a.lk / BK;
+    const int nk = a.lk / FBK;
     const int l16 = lane & 15, kq = lane >> 4;
 
-    d2_t ra[AP], rb[AP];
-    auto lstore = [&](int buf) {
-        double* la = sA[buf];
-        double* lb = sB[buf];
-#pragma unroll
-        for (int s = 0; s < AP; s++) {
-            double* q = la + (a_k + A_KR * s) * RS + a_m;
-            q[0] = ra[s][0];
-            q[1] = ra[s][1];
-        }
-#pragma unroll
-        for (int s = 0; s < AP; s++) {
-            double* q = lb + b_k * RS + b_n + B_NC * s;
-            q[0] = rb[s][0];
-            q[RS] = rb[s][1];
-        }
-    };
-    double fa[2][FI], fb[2][FJ];
+    pair_t ra[AP], rb[AP];
+    TL fa[2][FI], fb[2][FJ];
     auto frag = [&](int buf, int ks, int f) {
-        const double* la = sA[buf];
-        const double* lb = sB[buf];
+        const TL* la = sA[buf];
+        const TL* lb = sB[buf];
         const int kk = 4 * ks + kq;
 #pragma unroll
         for (int i = 0; i < FI; i++) fa[f][i] = la[kk * RS + wm + 16 * i + l16];
@@ -643,25 +635,48 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
 
     for (int r = r_begin; r < r_end; r++) {
         const fmm_entry_t ea = ent[uptr[r]], eb = ent[vptr[r]];
-        const double* pa = (const double*)a.A + ea.off + m0 + a_m + (long)a_k * lda;
-        const double* pb = (const double*)a.B + eb.off + (long)(n0 + b_n) * ldb + b_k;
-        const long da = (long)BK * lda;
+        const double ca = ea.coef, cb = eb.coef;
+        const TI* pa = (const TI*)a.A + ea.off + m0 + a_m + (long)a_k * lda;
+        const TI* pb = (const TI*)a.B + eb.off + (long)(n0 + b_n) * ldb + b_k;
+        const long da = (long)FBK * lda;
         auto gload = [&]() {
 #pragma unroll
-            for (int s = 0; s < AP; s++) ra[s] = *(const d2_t*)(pa + (long)(A_KR * s) * lda);
+            for (int s = 0; s < AP; s++) ra[s] = *(const pair_t*)(pa + (long)(A_KR * s) * lda);
 #pragma unroll
-            for (int s = 0; s < AP; s++) rb[s] = *(const d2_t*)(pb + (long)(B_NC * s) * ldb);
+            for (int s = 0; s < AP; s++) rb[s] = *(const pair_t*)(pb + (long)(B_NC * s) * ldb);
             pa += da;
-            pb += BK;
+            pb += FBK;
         };
-        auto scale = [&]() {
-            if (ea.coef != 1.0) {
+        // scaled as gemm_tile's staging does: in FP64, only when the
+        // coefficient is not 1 (a uniform branch), then rounded to the leaf type
+        auto lstore = [&](int buf) {
+            if (ca != 1.0) {
 #pragma unroll
-                for (int s = 0; s < AP; s++) ra[s] *= ea.coef;
+                for (int s = 0; s < AP; s++) {
+                    ra[s][0] = (TI)((double)ra[s][0] * ca);
+                    ra[s][1] = (TI)((double)ra[s][1] * ca);
+                }
             }
-            if (eb.coef != 1.0) {
+            if (cb != 1.0) {
 #pragma unroll
-                for (int s = 0; s < AP; s++) rb[s] *= eb.coef;
+                for (int s = 0; s < AP; s++) {
+                    rb[s][0] = (TI)((double)rb[s][0] * cb);
+                    rb[s][1] = (TI)((double)rb[s][1] * cb);
+                }
+            }
+            TL* la = sA[buf];
+            TL* lb = sB[buf];
+#pragma unroll
+            for (int s = 0; s < AP; s++) {
+                TL* q = la + (a_k + A_KR * s) * RS + a_m;
+                q[0] = (TL)ra[s][0];
+                q[1] = (TL)ra[s][1];
+            }
+#pragma unroll
+            for (int s = 0; s < AP; s++) {
+                TL* q = lb + b_k * RS + b_n + B_NC * s;
+                q[0] = (TL)rb[s][0];
+                q[RS] = (TL)rb[s][1];
             }
         };
         acc_t acc[FI][FJ];
@@ -670,7 +685,6 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
 #pragma unroll
             for (int j = 0; j < FJ; j++) acc[i][j] = (acc_t){0, 0, 0, 0};
         gload();
-        scale();
         lstore(0);
         __syncthreads();
         frag(0, 0, 0);
@@ -685,7 +699,7 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
 #pragma unroll
                 for (int i = 0; i < FI; i++)
 #pragma unroll
-                    for (int j = 0; j < FJ; j++) acc[i][j] = Leaf<double>::mfma(fa[f][i], fb[f][j], acc[i][j]);
+                    for (int j = 0; j < FJ; j++) acc[i][j] = Leaf<TL>::mfma(fa[f][i], fb[f][j], acc[i][j]);
                 if (ks + 1 < KS) {
                     constexpr int NR = (FI + FJ) / 2, NM = FI * FJ;
 #pragma unroll
@@ -694,16 +708,13 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
                         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);         // LDS read
                     }
                 }
-                if (ks == KS - 1 && more) {
-                    scale();
-                    lstore(cur ^ 1);
-                }
+                if (ks == KS - 1 && more) lstore(cur ^ 1);
             }
             __syncthreads();
             if (more) frag(cur ^ 1, 0, 0);
         }
         // W epilogue, every tile inside every C part (unclipped form)
-        const int er0 = wm + Leaf<double>::ROW_LANE * kq;
+        const int er0 = wm + Leaf<TL>::ROW_LANE * kq;
         const int ec0 = wn + l16;
         for (int e = wptr[r]; e < wptr[r + 1]; e++) {
             const fmm_entry_t we = ent[e];
@@ -711,38 +722,40 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
             const bool first = ((we.first >> a.gbit) & 1) != 0;
             const double beta = split ? 0.0 : a.beta;
             const int cs32 = (int)a.csC;
-            double* ct = Cb + we.off + (long)(m0 + er0) + (long)(n0 + ec0) * a.csC;
+            TO* ct = Cb + we.off + (long)(m0 + er0) + (long)(n0 + ec0) * a.csC;
             const bool rd = !first || beta != 0.0;
 #pragma unroll
             for (int j = 0; j < FJ; j++)
-                epi_block<false, FI, Leaf<double>::ROW_REG, true>(ct + 16 * j * cs32, 1, 0, 0, acc, j, lam, first, beta,
-                                                                  rd);
+                epi_block<false, FI, Leaf<TL>::ROW_REG, true>(ct + 16 * j * cs32, 1, 0, 0, acc, j, lam, first, beta,
+                                                              rd);
         }
     }
 }
 
-template <int UNUSED>   // a template, so each translation unit may instantiate it
+template <typename TI, typename TL, typename TO>
 __global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
 void fmm_fast_kernel(KArgs a)
 {
-    __shared__ double sA[2][lds_tile<double, FastCfg::T>()];
-    __shared__ double sB[2][lds_tile<double, FastCfg::T>()];
+    __shared__ TL sA[2][fast_lds<TL>()];
+    __shared__ TL sB[2][fast_lds<TL>()];
     // blocks [g*tiles, (g+1)*tiles) are product group g
     const int tiles = a.tiles_m * a.tiles_n;
     const int grp = (int)blockIdx.x / tiles;
     int tm, tn;
     tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
-    fast_tile(a, tm, tn, grp, sA, sB);
+    fast_tile<TI, TL, TO>(a, tm, tn, grp, sA, sB);
 }
 
-// Whether a prepared FP64 launch can take fmm_fast_kernel (single-part
-// operands are the caller's to guarantee).
+// Whether a prepared launch with leaf type TL can take fmm_fast_kernel
+// (single-part operands are the caller's to guarantee).
+template <typename TL>
 inline bool fast_eligible(const KArgs& ka, bool a_mn, bool b_mn, bool c_col)
 {
     constexpr int T = FastCfg::T;
     return a_mn && !b_mn && c_col && ka.va && ka.vb && !ka.ksplit && ka.lm % T == 0 && ka.ln % T == 0 &&
-           ka.lk % BK == 0 && ka.lk >= BK && ka.min_m == ka.lm && ka.min_n == ka.ln && ka.min_k == ka.lk &&
-           ka.min_cm >= ka.lm && ka.min_cn >= ka.ln && ka.tiles_m * T == ka.lm && ka.tiles_n * T == ka.ln;
+           ka.lk % fast_bk<TL>() == 0 && ka.lk >= fast_bk<TL>() && ka.min_m == ka.lm && ka.min_n == ka.ln &&
+           ka.min_k == ka.lk && ka.min_cm >= ka.lm && ka.min_cn >= ka.ln && ka.tiles_m * T == ka.lm &&
+           ka.tiles_n * T == ka.ln;
 }
 
 // Batched persistent kernel: a fixed grid of workgroups takes work items
@@ -801,13 +814,13 @@ void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restri
 
 // The batched form of the fast kernel (problems the host found eligible;
 // one product group per tile).
-template <int UNUSED>
+template <typename TI, typename TL, typename TO>
 __global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
 void fmm_batch_fast_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items,
                            const int* __restrict__ qoff, int* __restrict__ heads)
 {
-    __shared__ double sA[2][lds_tile<double, FastCfg::T>()];
-    __shared__ double sB[2][lds_tile<double, FastCfg::T>()];
+    __shared__ TL sA[2][fast_lds<TL>()];
+    __shared__ TL sB[2][fast_lds<TL>()];
     __shared__ int slot;
     int q = (int)blockIdx.x % kBatchQueues, tries = 0;
     for (;;) {
@@ -816,7 +829,7 @@ void fmm_batch_fast_kernel(const KArgs* __restrict__ probs, const BatchItem* __r
         const BatchItem it = items[w];
         const KArgs a = probs[it.problem];
         const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
-        fast_tile(a, tm, tn, 0, sA, sB);
+        fast_tile<TI, TL, TO>(a, tm, tn, 0, sA, sB);
         __syncthreads();                       // LDS free before the next item's prologue
     }
 }
@@ -878,14 +891,10 @@ int launch_typed(const KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, i
         return launch_layouts<TI, TL, TO, 1, false>(ka, a_mn, b_mn, c_col, nwg, st);
     }
     if (shape == 3) {
-        // the fast FP64 kernel (the host selects it for eligible single-part calls)
-        if constexpr (sizeof(TI) == 8 && sizeof(TL) == 8 && sizeof(TO) == 8) {
-            if (!fast_eligible(ka, a_mn, b_mn, c_col)) return FMMHIP_ERR_UNSUPPORTED;
-            hipLaunchKernelGGL(fmm_fast_kernel<0>, dim3(nwg), dim3(kFastNT), 0, st, ka);
-            return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
-        } else {
-            return FMMHIP_ERR_UNSUPPORTED;
-        }
+        // the fast kernel (the host selects it for eligible single-part calls)
+        if (!fast_eligible<TL>(ka, a_mn, b_mn, c_col)) return FMMHIP_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((fmm_fast_kernel<TI, TL, TO>), dim3(nwg), dim3(kFastNT), 0, st, ka);
+        return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
     }
     if (vec) return launch_layouts<TI, TL, TO, 0, true>(ka, a_mn, b_mn, c_col, nwg, st);
     return launch_layouts<TI, TL, TO, 0, false>(ka, a_mn, b_mn, c_col, nwg, st);
@@ -906,10 +915,8 @@ int launch_batch_typed(const KArgs* probs, const BatchItem* items, const int* qo
         if (vec) FMMK_BATCH(2, true);
         else FMMK_BATCH(2, false);
     } else if (shape == 3) {
-        if constexpr (sizeof(TI) == 8 && sizeof(TL) == 8 && sizeof(TO) == 8)
-            hipLaunchKernelGGL(fmm_batch_fast_kernel<0>, dim3(nwg), dim3(kFastNT), 0, st, probs, items, qoff, heads);
-        else
-            return FMMHIP_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((fmm_batch_fast_kernel<TI, TL, TO>), dim3(nwg), dim3(kFastNT), 0, st, probs, items, qoff,
+                           heads);
     } else {
         if (vec) FMMK_BATCH(0, true);
         else FMMK_BATCH(0, false);
