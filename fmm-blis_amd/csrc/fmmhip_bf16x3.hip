@@ -57,6 +57,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes, a native vector
+typedef __attribute__((address_space(3))) void lds_void_t;
+template <int I> struct IntC { static constexpr int value = I; };
 
 constexpr int KSTEP = 32;        // k per MFMA stage
 constexpr int CHUNK = 256;       // k per exact FP32 chunk
@@ -259,13 +261,16 @@ struct GCfg {
 __device__ __forceinline__ int lds_chunk(int row, int c) { return row * 4 + (c ^ ((row >> 1) & 3)); }
 
 template <int WM, int WN, int FI, int FJ, int PLANES, int TERMS>
-__global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
+__global__ __attribute__((amdgpu_flat_work_group_size(WM * WN * 64, WM * WN * 64),
+                          amdgpu_waves_per_eu(WM * WN / 4, WM * WN / 4))) void bf16x3_gemm_kernel(GemmArgs a)
 {
     static_assert((PLANES == 3 && (TERMS == 6 || TERMS == 8)) || (PLANES == 4 && TERMS == 10), "plane products");
     typedef GCfg<WM, WN, FI, FJ, PLANES> G;
     constexpr int BM = G::BM, BN = G::BN, NT = G::NT;
-    __shared__ u32x4 sA[2][G::A_CH];
-    __shared__ u32x4 sB[2][G::B_CH];
+    // two LDS stages as separate objects, so that the compiler tells the
+    // direct-to-LDS writes into one from the fragment reads of the other
+    __shared__ u32x4 sA0[G::A_CH], sA1[G::A_CH];
+    __shared__ u32x4 sB0[G::B_CH], sB1[G::B_CH];
     __shared__ int sE[2][BM + BN];
 
     // blocks [g*tiles, (g+1)*tiles) are product group g
@@ -280,41 +285,45 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % WM) * (FI * 16), wn = (wave / WM) * (FJ * 16);
 
-    // LDS chunk of the thread's staging row/chunk (the row swizzle depends on tid only)
-    const int st_base = (tid >> 2) * 4 + ((tid & 3) ^ ((tid >> 3) & 3));   // lds_chunk(tid >> 2, tid & 3)
-    // One k step of the A and B planes in flight from HBM to LDS.  Thread
-    // tid moves 16-byte chunk tid + NT*i of every stage; with (rows*4) % NT
-    // == 0 the chunk's plane is i / P and its row (tid >> 2) + NT (i % P) / 4,
-    // so every address is a per-thread base plus a compile-time offset.  The
-    // staging arrays are plain locals indexed in unrolled loops (small enough
-    // to be kept in registers as vectors).
-    constexpr int PA = BM * 4 / NT, PB = BN * 4 / NT;
-    u32x4 ra[G::A_PER];
-    u32x4 rb[G::B_PER];
+    // One k step of the A and B planes goes from HBM straight into an LDS
+    // stage: 2 x 8 PLANES moves of 1 KB (16 rows x 64 bytes of one plane),
+    // each one global_load_lds_dwordx4 in which lane i writes bytes
+    // [16 i, 16 i + 16) of the move's LDS block, i.e. row i / 4 of the block,
+    // stored chunk i % 4, which holds the row's k chunk (i % 4) ^ ((i >> 3) & 3)
+    // (the swizzle of lds_chunk).  Move q covers plane q / 8, rows 16 (q % 8)
+    // .. + 15; wave w makes moves MPW w .. MPW w + MPW - 1 of each operand
+    // (MPW = 8 PLANES / waves).  No registers and no LDS store instructions are involved; the
+    // moves count against vmcnt.  The chunk exponents go through a register.
+    static_assert(BM == 128 && BN == 128, "eight 16-row moves per plane and operand");
+    constexpr int MPW = 8 * PLANES / (NT / 64);   // moves per wave and operand
+    static_assert(MPW * (NT / 64) == 8 * PLANES, "the moves divide among the waves");
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int lane_off = (lane >> 2) * KSTEP + 8 * ((lane & 3) ^ ((lane >> 3) & 3));
     int re = 0;
-#define BF3_GLOAD(R_, KB_)                                                                                   \
-    do {                                                                                                     \
-        const int r_ = (R_), kb_ = (KB_);                                                                    \
-        const u16* Ab_ = a.A + (long)r_ * PLANES * a.a_plane + ((long)kb_ * a.LpA + m0) * KSTEP + tid * 8;   \
-        const u16* Bb_ = a.B + (long)r_ * PLANES * a.b_plane + ((long)kb_ * a.LpB + n0) * KSTEP + tid * 8;   \
-        _Pragma("unroll") for (int i_ = 0; i_ < G::A_PER; i_++)                                              \
-            ra[i_] = *(const u32x4*)(Ab_ + (i_ / PA) * a.a_plane + (i_ % PA) * NT * 8);                       \
-        _Pragma("unroll") for (int i_ = 0; i_ < G::B_PER; i_++)                                              \
-            rb[i_] = *(const u32x4*)(Bb_ + (i_ / PB) * a.b_plane + (i_ % PB) * NT * 8);                       \
-        if (kb_ % STEPS_PER_CHUNK == 0) {                                                                    \
-            const int ch_ = kb_ / STEPS_PER_CHUNK;                                                           \
-            if (tid < BM) re = a.EA[(long)r_ * a.ea_prod + (long)ch_ * a.LpA + m0 + tid];                    \
-            else if (tid < BM + BN) re = a.EB[(long)r_ * a.eb_prod + (long)ch_ * a.LpB + n0 + tid - BM];     \
-        }                                                                                                    \
-    } while (0)
-#define BF3_LSTORE(ST_, CHUNK_START_, EP_)                                                                   \
-    do {                                                                                                     \
-        _Pragma("unroll") for (int i_ = 0; i_ < G::A_PER; i_++)                                              \
-            sA[ST_][st_base + (i_ / PA) * BM * 4 + (i_ % PA) * NT] = ra[i_];                                 \
-        _Pragma("unroll") for (int i_ = 0; i_ < G::B_PER; i_++)                                              \
-            sB[ST_][st_base + (i_ / PB) * BN * 4 + (i_ % PB) * NT] = rb[i_];                                 \
-        if ((CHUNK_START_) && tid < BM + BN) sE[EP_][tid] = re;                                              \
-    } while (0)
+    auto stage_in = [&](u32x4* dA, u32x4* dB, int r_, int kb_) {
+        const u16* Ab_ = a.A + (long)r_ * PLANES * a.a_plane + ((long)kb_ * a.LpA + m0) * KSTEP + lane_off;
+        const u16* Bb_ = a.B + (long)r_ * PLANES * a.b_plane + ((long)kb_ * a.LpB + n0) * KSTEP + lane_off;
+#pragma unroll
+        for (int t = 0; t < MPW; t++) {
+            const int q = MPW * wv + t, g = q >> 3, rb = q & 7;
+            __builtin_amdgcn_global_load_lds((const void*)(Ab_ + g * a.a_plane + rb * 16 * KSTEP),
+                                             (lds_void_t*)(dA + 64 * q), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(Bb_ + g * a.b_plane + rb * 16 * KSTEP),
+                                             (lds_void_t*)(dB + 64 * q), 16, 0, 0);
+        }
+        if (kb_ % STEPS_PER_CHUNK == 0) {
+            const int ch_ = kb_ / STEPS_PER_CHUNK;
+            if (tid < BM) re = a.EA[(long)r_ * a.ea_prod + (long)ch_ * a.LpA + m0 + tid];
+            else if (tid < BM + BN) re = a.EB[(long)r_ * a.eb_prod + (long)ch_ * a.LpB + n0 + tid - BM];
+        }
+    };
+    // the moves of this wave (and its exponent load) have landed, the
+    // exponents are stored, then every wave's moves have landed
+    auto stage_done = [&](bool chunk_start, int ep) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        if (chunk_start && tid < BM + BN) sE[ep][tid] = re;
+        __syncthreads();
+    };
     floatx4 G0[FI][FJ], G1[FI][FJ], G2[FI][FJ];
     double acc[FI][FJ][4];
 #pragma unroll
@@ -326,31 +335,33 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
             for (int q = 0; q < 4; q++) acc[i][j][q] = 0.0;
         }
 
-    BF3_GLOAD(r_begin, 0);
-    BF3_LSTORE(0, true, 0);
-    __syncthreads();
+    stage_in(sA0, sB0, r_begin, 0);
+    stage_done(true, 0);
 
     // fragment reads: row wm + 16 i + (lane & 15) (resp. wn + 16 j + ...),
     // 8-wide k block kc = lane >> 4; the swizzle depends on lane & 15 only
     const int kc = lane >> 4;
     const int rd_a = (wm + (lane & 15)) * 4 + (kc ^ (((lane & 15) >> 1) & 3));
     const int rd_b = (wn + (lane & 15)) * 4 + (kc ^ (((lane & 15) >> 1) & 3));
-    int st = 0;           // LDS stage of the current step
     int chunk = 0;        // chunks completed (exponent buffer parity)
-    for (int r = r_begin; r < r_end; r++) {
-        for (int kb = 0; kb < a.nkb; kb++) {
-            int rn = r, kbn = kb + 1;
-            if (kbn == a.nkb) { rn = r + 1; kbn = 0; }
-            const bool more = rn < r_end;
-            if (more) BF3_GLOAD(rn, kbn);
+    // One k step of product r on LDS stage S_ (kb & 1: nkb is a multiple of
+    // STEPS_PER_CHUNK, so every product starts on stage 0) while the next
+    // step's planes move into the other stage (read in the previous step,
+    // whose barrier every wave has passed).
+    auto step = [&](int r, int kb, auto stage) {
+        constexpr int S_ = decltype(stage)::value;
+        int rn = r, kbn = kb + 1;
+        if (kbn == a.nkb) { rn = r + 1; kbn = 0; }
+        const bool more = rn < r_end;
+        if (more) stage_in(S_ ? sA0 : sA1, S_ ? sB0 : sB1, rn, kbn);
 
             // the plane products of the step; with four planes LLVM's
             // MFMA/LDS-read interleaving strategy 0 (3% faster than none at
             // 16384^3, classical and 222 L2; strategies 1 and 2 measured 0-4%
             // apart from it); the three-plane forms run 2.6x slower with it
             if (PLANES == 4) __builtin_amdgcn_iglp_opt(0);
-            const u32x4* la = sA[st];
-            const u32x4* lb = sB[st];
+            const u32x4* la = S_ ? sA1 : sA0;
+            const u32x4* lb = S_ ? sB1 : sB0;
             auto fa = [&](int g, int i) { return __builtin_bit_cast(bf16x8, la[rd_a + (g * BM + 16 * i) * 4]); };
             auto fb = [&](int g, int j) { return __builtin_bit_cast(bf16x8, lb[rd_b + (g * BN + 16 * j) * 4]); };
 #define BF3_MFMA(ACC, X, Y) ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, Y, ACC, 0, 0, 0)
@@ -523,13 +534,14 @@ __global__ __launch_bounds__(WM * WN * 64) void bf16x3_gemm_kernel(GemmArgs a)
 #pragma unroll
                         for (int q = 0; q < 4; q++) acc[i][j][q] = 0.0;
             }
-            if (more) BF3_LSTORE(st ^ 1, kbn % STEPS_PER_CHUNK == 0, chunk & 1);
-            st ^= 1;
-            __syncthreads();
+        stage_done(more && kbn % STEPS_PER_CHUNK == 0, chunk & 1);
+    };
+    for (int r = r_begin; r < r_end; r++) {
+        for (int kb = 0; kb < a.nkb; kb += 2) {
+            step(r, kb, IntC<0>());
+            step(r, kb + 1, IntC<1>());
         }
     }
-#undef BF3_GLOAD
-#undef BF3_LSTORE
 }
 
 }  // namespace bf3
