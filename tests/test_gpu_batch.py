@@ -251,6 +251,49 @@ def test_fast_kernel_equals_shape0(case):
             assert np.array_equal(outs[0], want), case
 
 
+# (dtype or leaf, m, n, k, cand, pack): FP32 operands and the FP32 leaf on
+# leaves the fast kernel takes, with 1, 2, 3 and 8 of its 32-wide k-steps
+# (its k-step order peels the last two steps)
+FAST32 = [("f4", 256, 256, 32, 0, -1), ("f4", 256, 384, 64, 0, -1), ("f4", 384, 256, 96, 0, -1),
+          ("f4", 512, 512, 1024, 2, 1), ("f4", 512, 512, 384, 2, 2), ("f8/f32", 256, 256, 32, 0, -1),
+          ("f8/f32", 256, 256, 64, 0, -1), ("f8/f32", 512, 512, 256, 2, 1), ("f8/f32", 512, 512, 128, 2, 2)]
+
+
+@pytest.mark.parametrize("case", FAST32, ids=lambda c: "-".join(map(str, c)).replace("/", "_"))
+def test_fast_kernel_fp32_equals_shape0(case):
+    """FP32 operands (sgemm) and FP64 operands with the FP32 leaf: the
+    automatic shape takes the fast kernel, whose C equals shape 0's bit for
+    bit; integer inputs equal the oracle."""
+    kind, m, n, k, cand, pack = case
+    i = FAST32.index(case)
+    dt = "f4" if kind == "f4" else "f8"
+    for integer in (False, True):
+        A, B, C = inputs(dt, m, n, k, 80 + i, integer)
+        alpha, beta = (0.7, -1.3) if not integer else (2.0, -1.0)
+        dA, dB = dev(A), dev(B)
+        outs, shapes = [], []
+        for shape in (-1, 0):
+            alg = alg_for(cand)
+            if cand > 0:
+                alg.set_pack(pack)
+            alg.set_shape(shape)
+            dC = dev(C)
+            if kind == "f4":
+                F.sgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC)
+            else:
+                F.dgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC, leaf=F.LEAF_F32)
+            F.sync()
+            outs.append(dC.to_numpy())
+            shapes.append(alg.last_shape())
+            alg.close()
+        assert shapes == [3, 0], (case, shapes)
+        assert np.array_equal(outs[0], outs[1]), (case, integer)
+        if integer:
+            A64, B64, C64 = (np.asfortranarray(X.astype(np.float64)) for X in (A, B, C))
+            want = O.dgemm(alpha, A64, B64, beta, C64.copy(order="F"))
+            assert np.array_equal(outs[0].astype(np.float64), want), case
+
+
 def test_batch_graph_replays_with_eager_work_between():
     b, probs = build(integer=False)
     b.plan()
