@@ -144,6 +144,17 @@ template <> struct Leaf<float> {
     }
 };
 
+// Global-memory views of pointers.  The batched kernels read a problem's
+// arguments from memory, where the compiler cannot tell the address space
+// of the pointers in them and emits flat loads and stores; flat ones also
+// count against lgkmcnt, which ties the waits for the LDS fragment reads to
+// the global loads of the next tile.  Loads and stores through these types
+// are global ones (the pointers do point to global memory).
+template <typename T> struct GPtr {
+    typedef const __attribute__((address_space(1))) T c;
+    typedef __attribute__((address_space(1))) T m;
+};
+
 template <typename TI> struct Pair;
 template <> struct Pair<double> { typedef d2_t t; };
 template <> struct Pair<float> { typedef f2_t t; };
@@ -191,15 +202,17 @@ struct Stage {
     __device__ __forceinline__ void load(const TI* __restrict__ base, const fmm_entry_t& e, long ld,
                                          int mn0, int k0, bool interior, double (&x)[PAIRS][2]) const
     {
-        const TI* t = base + e.off + (MN_CONTIG ? (long)mn0 + (long)k0 * ld : (long)mn0 * ld + k0) + off0;
         typedef typename Pair<TI>::t pair_t;
+        typedef typename GPtr<TI>::c gTI;
+        typedef typename GPtr<pair_t>::c gpair_t;
+        gTI* t = (gTI*)base + e.off + (MN_CONTIG ? (long)mn0 + (long)k0 * ld : (long)mn0 * ld + k0) + off0;
         const int ld32 = (int)ld;
         if (interior) {
 #pragma unroll
             for (int s = 0; s < PAIRS; s++) {
-                const TI* p = t + (MN_CONTIG ? dk(s) * ld32 : dmn(s) * ld32);
+                gTI* p = t + (MN_CONTIG ? dk(s) * ld32 : dmn(s) * ld32);
                 if (VEC) {
-                    pair_t q = *(const pair_t*)p;
+                    pair_t q = *(gpair_t*)p;
                     x[s][0] = (double)q[0];
                     x[s][1] = (double)q[1];
                 } else {
@@ -213,14 +226,14 @@ struct Stage {
         const int ek = (IS_B ? e.rows : e.cols) - k0;
 #pragma unroll
         for (int s = 0; s < PAIRS; s++) {
-            const TI* p = t + (MN_CONTIG ? dk(s) * ld32 : dmn(s) * ld32);
+            gTI* p = t + (MN_CONTIG ? dk(s) * ld32 : dmn(s) * ld32);
             const int mn_0 = lmn0 + dmn(s), k_0 = lk0 + dk(s);
             // element 1 is one step along the contiguous dimension
             const bool ok0 = mn_0 < emn && k_0 < ek;
             const bool ok1 = MN_CONTIG ? (mn_0 + 1 < emn && k_0 < ek) : (mn_0 < emn && k_0 + 1 < ek);
             double x0 = 0.0, x1 = 0.0;
             if (VEC && ok1) {
-                pair_t q = *(const pair_t*)p;
+                pair_t q = *(gpair_t*)p;
                 x0 = (double)q[0];
                 x1 = (double)q[1];
             } else {
@@ -310,13 +323,14 @@ __device__ __forceinline__ void epi_block(TO* __restrict__ cb, int rs32, int ero
                                           const ACC (&acc)[FI][FJ], int j, double lam, bool first, double beta,
                                           bool rd)
 {
+    typename GPtr<TO>::m* g = (typename GPtr<TO>::m*)cb;
     double v[FI][4];
 #pragma unroll
     for (int i = 0; i < FI; i++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int dr = 16 * i + ROW_REG * q;
-            const TO* p = cb + (C_COL ? dr : dr * rs32);
+            typename GPtr<TO>::c* p = g + (C_COL ? dr : dr * rs32);
             v[i][q] = (rd && (!CHECK || (dr < erows && 0 < ecols))) ? (double)*p : 0.0;
         }
 #pragma unroll
@@ -328,7 +342,7 @@ __device__ __forceinline__ void epi_block(TO* __restrict__ cb, int rs32, int ero
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int dr = 16 * i + ROW_REG * q;
-            if (!CHECK || (dr < erows && 0 < ecols)) cb[C_COL ? dr : dr * rs32] = (TO)v[i][q];
+            if (!CHECK || (dr < erows && 0 < ecols)) g[C_COL ? dr : dr * rs32] = (TO)v[i][q];
         }
 }
 
@@ -589,6 +603,21 @@ __device__ __host__ constexpr int fast_bk() { return sizeof(TL) == 8 ? 16 : 32; 
 template <typename TL>
 __device__ __host__ constexpr int fast_lds() { return fast_bk<TL>() * lds_row<TL, FastCfg::T>(); }
 
+// An entry read field by field through a global-memory pointer (a struct
+// copy from another address space has no constructor in C++).
+typedef typename GPtr<fmm_entry_t>::c gent_t;
+__device__ __forceinline__ fmm_entry_t load_entry(gent_t* p)
+{
+    fmm_entry_t e;
+    e.off = p->off;
+    e.rows = p->rows;
+    e.cols = p->cols;
+    e.coef = p->coef;
+    e.first = p->first;
+    e.part = p->part;
+    return e;
+}
+
 // One tile position (tm, tn) of product group grp: the body of the fast
 // kernel and of its batched persistent form.
 template <typename TI, typename TL, typename TO>
@@ -612,13 +641,18 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
     const int b_k = 2 * (tid % B_KP), b_n = tid / B_KP;
     const long lda = a.lda, ldb = a.ldb;
     const bool split = a.groups > 1;
+    // operands and lists as global-memory pointers (the batched form reads
+    // the arguments from memory; see GPtr)
+    typedef typename GPtr<TI>::c gTI;
+    typedef typename GPtr<pair_t>::c gpair_t;
     TO* __restrict__ Cb = split ? (TO*)a.tmp + (long)grp * a.tmp_elems : (TO*)a.C;
     const int R = a.R;
     const int r_begin = (int)((long)grp * R / a.groups), r_end = (int)((long)(grp + 1) * R / a.groups);
-    const int* __restrict__ uptr = a.ptr;
-    const int* __restrict__ vptr = a.ptr + (R + 1);
-    const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
-    const fmm_entry_t* __restrict__ ent = a.ent;
+    typedef typename GPtr<int>::c gint;
+    gint* __restrict__ uptr = (gint*)a.ptr;
+    gint* __restrict__ vptr = uptr + (R + 1);
+    gint* __restrict__ wptr = uptr + 2 * (R + 1);
+    gent_t* __restrict__ ent = (gent_t*)a.ent;
     const int nk = a.lk / FBK;
     const int l16 = lane & 15, kq = lane >> 4;
 
@@ -635,16 +669,16 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
     };
 
     for (int r = r_begin; r < r_end; r++) {
-        const fmm_entry_t ea = ent[uptr[r]], eb = ent[vptr[r]];
+        const fmm_entry_t ea = load_entry(ent + uptr[r]), eb = load_entry(ent + vptr[r]);
         const double ca = ea.coef, cb = eb.coef;
-        const TI* pa = (const TI*)a.A + ea.off + m0 + a_m + (long)a_k * lda;
-        const TI* pb = (const TI*)a.B + eb.off + (long)(n0 + b_n) * ldb + b_k;
+        gTI* pa = (gTI*)(const TI*)a.A + ea.off + m0 + a_m + (long)a_k * lda;
+        gTI* pb = (gTI*)(const TI*)a.B + eb.off + (long)(n0 + b_n) * ldb + b_k;
         const long da = (long)FBK * lda;
         auto gload = [&]() {
 #pragma unroll
-            for (int s = 0; s < AP; s++) ra[s] = *(const pair_t*)(pa + (long)(A_KR * s) * lda);
+            for (int s = 0; s < AP; s++) ra[s] = *(gpair_t*)(pa + (long)(A_KR * s) * lda);
 #pragma unroll
-            for (int s = 0; s < AP; s++) rb[s] = *(const pair_t*)(pb + (long)(B_NC * s) * ldb);
+            for (int s = 0; s < AP; s++) rb[s] = *(gpair_t*)(pb + (long)(B_NC * s) * ldb);
             pa += da;
             pb += FBK;
         };
@@ -782,7 +816,7 @@ __device__ __forceinline__ void fast_tile(const KArgs& a, int tm, int tn, int gr
         const int er0 = wm + Leaf<TL>::ROW_LANE * kq;
         const int ec0 = wn + l16;
         for (int e = wptr[r]; e < wptr[r + 1]; e++) {
-            const fmm_entry_t we = ent[e];
+            const fmm_entry_t we = load_entry(ent + e);
             const double lam = a.alpha * we.coef;
             const bool first = ((we.first >> a.gbit) & 1) != 0;
             const double beta = split ? 0.0 : a.beta;
