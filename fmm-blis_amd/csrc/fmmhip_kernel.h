@@ -155,6 +155,21 @@ template <typename T> struct GPtr {
     typedef __attribute__((address_space(1))) T m;
 };
 
+// An entry read field by field through a global-memory pointer (a struct
+// copy from another address space has no constructor in C++).
+typedef typename GPtr<fmm_entry_t>::c gent_t;
+__device__ __forceinline__ fmm_entry_t load_entry(gent_t* p)
+{
+    fmm_entry_t e;
+    e.off = p->off;
+    e.rows = p->rows;
+    e.cols = p->cols;
+    e.coef = p->coef;
+    e.first = p->first;
+    e.part = p->part;
+    return e;
+}
+
 template <typename TI> struct Pair;
 template <> struct Pair<double> { typedef d2_t t; };
 template <> struct Pair<float> { typedef f2_t t; };
@@ -378,10 +393,13 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
     constexpr bool ks = KSPL;
     const int r_begin = ks ? 0 : (int)((long)grp * R / a.groups);
     const int r_end = ks ? R : (int)((long)(grp + 1) * R / a.groups);
-    const int* __restrict__ uptr = a.ptr;
-    const int* __restrict__ vptr = a.ptr + (R + 1);
-    const int* __restrict__ wptr = a.ptr + 2 * (R + 1);
-    const fmm_entry_t* __restrict__ ent = a.ent;
+    // lists as global-memory pointers (the batched form reads the arguments
+    // from memory; see GPtr)
+    typedef typename GPtr<int>::c gint;
+    gint* __restrict__ uptr = (gint*)a.ptr;
+    gint* __restrict__ vptr = uptr + (R + 1);
+    gint* __restrict__ wptr = uptr + 2 * (R + 1);
+    gent_t* __restrict__ ent = (gent_t*)a.ent;
     const int nk = (a.lk + BK - 1) / BK;
     const int kt_begin = ks ? (int)((long)grp * nk / a.groups) : 0;
     const int kt_end = ks ? (int)((long)(grp + 1) * nk / a.groups) : nk;
@@ -403,7 +421,7 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
         const int v0 = vptr[r], nv = vptr[r + 1] - v0;
         const int ng = nu > nv ? nu : nv;   // load groups: part g of A and part g of B
         // the product's first parts, held for all k-steps
-        const fmm_entry_t ea0 = ent[u0], eb0 = ent[v0];
+        const fmm_entry_t ea0 = load_entry(ent + u0), eb0 = load_entry(ent + v0);
 
         acc_t acc[FI][FJ];
 #pragma unroll
@@ -426,11 +444,11 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
             stA.scale_first(ea0.coef);
             stB.scale_first(eb0.coef);
             for (int g = 1; g < nu; g++) {
-                stA.template load_tmp<VEC>(Ab, ent[u0 + g], lda, m0, kp, ia);
+                stA.template load_tmp<VEC>(Ab, load_entry(ent + u0 + g), lda, m0, kp, ia);
                 stA.add_tmp(ent[u0 + g].coef);
             }
             for (int g = 1; g < nv; g++) {
-                stB.template load_tmp<VEC>(Bb, ent[v0 + g], ldb, n0, kp, ib);
+                stB.template load_tmp<VEC>(Bb, load_entry(ent + v0 + g), ldb, n0, kp, ib);
                 stB.add_tmp(ent[v0 + g].coef);
             }
             stA.template store<TL>(sA[0]);
@@ -461,8 +479,8 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
                     stB.template load_first<VEC>(Bb, eb0, ldb, n0, k0, ib);
                 }
                 if (MULTI) {
-                    if (1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + 1], lda, m0, k0, ia);
-                    if (1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + 1], ldb, n0, k0, ib);
+                    if (1 < nu) stA.template load_tmp<VEC>(Ab, load_entry(ent + u0 + 1), lda, m0, k0, ia);
+                    if (1 < nv) stB.template load_tmp<VEC>(Bb, load_entry(ent + v0 + 1), ldb, n0, k0, ib);
                 }
             }
             const TL* la = sA[cur];
@@ -501,8 +519,8 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
                     if (g < ng) {
                         if (g < nu) stA.add_tmp(ent[u0 + g].coef);
                         if (g < nv) stB.add_tmp(ent[v0 + g].coef);
-                        if (g + 1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + g + 1], lda, m0, k0, ia);
-                        if (g + 1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + g + 1], ldb, n0, k0, ib);
+                        if (g + 1 < nu) stA.template load_tmp<VEC>(Ab, load_entry(ent + u0 + g + 1), lda, m0, k0, ia);
+                        if (g + 1 < nv) stB.template load_tmp<VEC>(Bb, load_entry(ent + v0 + g + 1), ldb, n0, k0, ib);
                     }
                 }
             }
@@ -512,8 +530,8 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
                     for (int g = KS + 1; g < ng; g++) {
                         if (g < nu) stA.add_tmp(ent[u0 + g].coef);
                         if (g < nv) stB.add_tmp(ent[v0 + g].coef);
-                        if (g + 1 < nu) stA.template load_tmp<VEC>(Ab, ent[u0 + g + 1], lda, m0, k0, ia);
-                        if (g + 1 < nv) stB.template load_tmp<VEC>(Bb, ent[v0 + g + 1], ldb, n0, k0, ib);
+                        if (g + 1 < nu) stA.template load_tmp<VEC>(Ab, load_entry(ent + u0 + g + 1), lda, m0, k0, ia);
+                        if (g + 1 < nv) stB.template load_tmp<VEC>(Bb, load_entry(ent + v0 + g + 1), ldb, n0, k0, ib);
                     }
                 } else {
                     // single: every further part after the MFMA block, one
@@ -521,11 +539,11 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
                     stA.scale_first(ea0.coef);
                     stB.scale_first(eb0.coef);
                     for (int g = 1; g < nu; g++) {
-                        stA.template load_tmp<VEC>(Ab, ent[u0 + g], lda, m0, k0, ia);
+                        stA.template load_tmp<VEC>(Ab, load_entry(ent + u0 + g), lda, m0, k0, ia);
                         stA.add_tmp(ent[u0 + g].coef);
                     }
                     for (int g = 1; g < nv; g++) {
-                        stB.template load_tmp<VEC>(Bb, ent[v0 + g], ldb, n0, k0, ib);
+                        stB.template load_tmp<VEC>(Bb, load_entry(ent + v0 + g), ldb, n0, k0, ib);
                         stB.add_tmp(ent[v0 + g].coef);
                     }
                 }
@@ -543,7 +561,7 @@ __device__ __forceinline__ void gemm_tile(const KArgs& a, int tm, int tn, int gr
         const int er0 = wm + Leaf<TL>::ROW_LANE * (lane >> 4);
         const int ec0 = wn + (lane & 15);
         for (int e = w0; e < w1; e++) {
-            const fmm_entry_t we = ent[e];
+            const fmm_entry_t we = load_entry(ent + e);
             const double lam = a.alpha * we.coef;
             const bool first = ((we.first >> a.gbit) & 1) != 0;
             const double beta = split ? 0.0 : a.beta;      // temporaries start at zero
@@ -602,21 +620,6 @@ template <typename TL>
 __device__ __host__ constexpr int fast_bk() { return sizeof(TL) == 8 ? 16 : 32; }
 template <typename TL>
 __device__ __host__ constexpr int fast_lds() { return fast_bk<TL>() * lds_row<TL, FastCfg::T>(); }
-
-// An entry read field by field through a global-memory pointer (a struct
-// copy from another address space has no constructor in C++).
-typedef typename GPtr<fmm_entry_t>::c gent_t;
-__device__ __forceinline__ fmm_entry_t load_entry(gent_t* p)
-{
-    fmm_entry_t e;
-    e.off = p->off;
-    e.rows = p->rows;
-    e.cols = p->cols;
-    e.coef = p->coef;
-    e.first = p->first;
-    e.part = p->part;
-    return e;
-}
 
 // One tile position (tm, tn) of product group grp: the body of the fast
 // kernel and of its batched persistent form.
