@@ -23,6 +23,27 @@
 
 namespace {
 
+// Global-memory views of pointers: the batched kernels read each segment's
+// pointers from memory, where the compiler cannot tell their address space
+// and would emit flat loads and stores (which also count against lgkmcnt);
+// loads and stores through these types are global ones.
+template <typename T> struct G1 {
+    typedef const __attribute__((address_space(1))) T c;
+    typedef __attribute__((address_space(1))) T m;
+};
+__device__ __forceinline__ fmm_entry_t entry_at(const fmm_entry_t* base, int i)
+{
+    typename G1<fmm_entry_t>::c* p = (typename G1<fmm_entry_t>::c*)base + i;
+    fmm_entry_t e;
+    e.off = p->off;
+    e.rows = p->rows;
+    e.cols = p->cols;
+    e.coef = p->coef;
+    e.first = p->first;
+    e.part = p->part;
+    return e;
+}
+
 // Thread (c0..c0+1, s) of the leaf; loops over all products.
 template <typename T>
 __global__ __launch_bounds__(256) void presum_kernel(const T* __restrict__ X, long ld, const int* __restrict__ ptr,
@@ -67,8 +88,9 @@ __global__ __launch_bounds__(256) void presum_batch_kernel(const fmmhip_presum_s
         if (boff[mid] <= b) lo = mid; else hi = mid - 1;
     }
     const fmmhip_presum_seg_t sg = segs[lo];
-    const T* __restrict__ X = (const T*)sg.X;
-    T* __restrict__ S = (T*)sg.S;
+    typename G1<T>::c* __restrict__ X = (typename G1<T>::c*)(const T*)sg.X;
+    typename G1<T>::m* __restrict__ S = (typename G1<T>::m*)(T*)sg.S;
+    typename G1<int>::c* __restrict__ sptr = (typename G1<int>::c*)sg.ptr;
     const long pairs_c = (sg.Lc + 1) / 2;
     const long total = pairs_c * sg.Ls;
     const long plane = (long)sg.ldo * sg.Ls;
@@ -80,16 +102,16 @@ __global__ __launch_bounds__(256) void presum_batch_kernel(const fmmhip_presum_s
         const int c = 2 * (int)(idx - (long)s * pairs_c);
         for (int r = 0; r < sg.R; r++) {
             double v0 = 0.0, v1 = 0.0;
-            for (int e = sg.ptr[r]; e < sg.ptr[r + 1]; e++) {
-                const fmm_entry_t en = sg.ent[e];
+            for (int e = sptr[r]; e < sptr[r + 1]; e++) {
+                const fmm_entry_t en = entry_at(sg.ent, e);
                 const int ec = sg.swap ? en.cols : en.rows;
                 const int es = sg.swap ? en.rows : en.cols;
                 if (s >= es) continue;
-                const T* p = X + en.off + c + (long)s * sg.ld;
+                typename G1<T>::c* p = X + en.off + c + (long)s * sg.ld;
                 if (c < ec) v0 = fma(en.coef, (double)p[0], v0);
                 if (c + 1 < ec) v1 = fma(en.coef, (double)p[1], v1);
             }
-            T* q = S + r * plane + c + (long)s * sg.ldo;
+            typename G1<T>::m* q = S + r * plane + c + (long)s * sg.ldo;
             q[0] = (T)v0;
             if (c + 1 < sg.Lc) q[1] = (T)v1;
         }
@@ -108,7 +130,7 @@ __device__ __forceinline__ void parts_meta(const fmm_entry_t* __restrict__ parts
                                            int* p_c, int* p_s)
 {
     for (int p = threadIdx.x; p < np; p += TB) {
-        const fmm_entry_t pe = parts[p];
+        const fmm_entry_t pe = entry_at(parts, p);
         p_off[p] = pe.off;
         p_c[p] = swap ? pe.cols : pe.rows;
         p_s[p] = swap ? pe.rows : pe.cols;
@@ -126,6 +148,9 @@ __device__ __forceinline__ void parts_unit(const T* __restrict__ X, long ld, con
                                            const long* p_off, const int* p_c, const int* p_s)
 {
     const int t = threadIdx.x;
+    typename G1<T>::c* __restrict__ Xg = (typename G1<T>::c*)X;
+    typename G1<T>::m* __restrict__ Sg = (typename G1<T>::m*)S;
+    typename G1<int>::c* __restrict__ ptrg = (typename G1<int>::c*)ptr;
     for (int p0 = 0; p0 < np; p0 += 16) {
         double xv[16];
 #pragma unroll
@@ -133,7 +158,7 @@ __device__ __forceinline__ void parts_unit(const T* __restrict__ X, long ld, con
             const int p = min(p0 + j, np - 1);
             const int ec = p_c[p], es = p_s[p];
             const long at = ((ec > 0) & (es > 0)) ? p_off[p] + min(c, ec - 1) + (long)min(s, es - 1) * ld : 0;
-            const double x = (double)X[at];
+            const double x = (double)Xg[at];
             xv[j] = ((c < ec) & (s < es)) ? x : 0.0;
         }
 #pragma unroll
@@ -143,11 +168,11 @@ __device__ __forceinline__ void parts_unit(const T* __restrict__ X, long ld, con
     if (c < Lc) {
         for (int r = 0; r < R; r++) {
             double v = 0.0;
-            for (int e = ptr[r]; e < ptr[r + 1]; e++) {
-                const fmm_entry_t en = ent[e];
+            for (int e = ptrg[r]; e < ptrg[r + 1]; e++) {
+                const fmm_entry_t en = entry_at(ent, e);
                 v = fma(en.coef, xs[en.part][t], v);
             }
-            S[r * plane + c + (long)s * ldo] = (T)v;
+            Sg[r * plane + c + (long)s * ldo] = (T)v;
         }
     }
 }
