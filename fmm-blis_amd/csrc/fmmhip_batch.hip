@@ -18,6 +18,30 @@
 #include "fmmhip_host.h"
 
 using fmmk::BatchItem;
+
+namespace {
+
+// Whether a problem prepared for the general 128 x 128 kernel can run its
+// interior tiles on the fast kernel: the fast kernel's layouts, 16-byte
+// aligned pairs, no k-split, every part with the leaf's full k in whole
+// k-steps of the leaf type (d: FP64).
+bool fast_tiles_ok(const Prepared& P, int d)
+{
+    const fmmk::KArgs& a = P.ka;
+    const int bk = d ? fmmk::fast_bk<double>() : fmmk::fast_bk<float>();
+    return P.a_mn && !P.b_mn && P.c_col && a.va && a.vb && !a.ksplit && a.groups == 1 && a.lk % bk == 0 &&
+           a.lk >= bk && a.min_k == a.lk;
+}
+
+// Whether tile (tm, tn) lies inside every part of A, B and C (the fast
+// kernel's loads and its unclipped W epilogue need no bounds there).
+bool fast_tile_inside(const fmmk::KArgs& a, long tm, long tn)
+{
+    constexpr long T = fmmk::FastCfg::T;
+    return (tm + 1) * T <= std::min(a.min_m, a.min_cm) && (tn + 1) * T <= std::min(a.min_n, a.min_cn);
+}
+
+}  // namespace
 using fmmk::KArgs;
 
 namespace {
@@ -518,7 +542,8 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             if (b->prep[i].presum) bind_workspace(&b->prep[i], (char*)b->ws + ws_off[i]);
     }
     // 2. per datatype and launch shape (group g = 2 * s + datatype, s = 0
-    //    for 128 x 128 tiles, 1 for 64 x 64, 2 for the fast FP64 kernel):
+    //    for 128 x 128 tiles, 1 for 64 x 64, 2 for the fast kernel: whole
+    //    problems, and the interior tiles of problems on 128 x 128 tiles):
     //    kernel arguments and work items (costliest first); per datatype:
     //    summation segments and their block offsets
     constexpr int kShapes = 3;
@@ -544,6 +569,15 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         const int pi = (int)ka[g].size();
         ka[g].push_back(P.ka);
         vec[g] = vec[g] && P.ka.va && P.ka.vb;
+        // a problem on the general 128 x 128 kernel sends the tiles that lie
+        // inside all its parts to the fast kernel (same accumulation order,
+        // so the same bits); the edge tiles stay on the general kernel
+        const int gf = 2 * 2 + d;
+        int pf = -1;
+        if (g == d && P.single_part && fast_tiles_ok(P, d)) {
+            pf = (int)ka[gf].size();
+            ka[gf].push_back(P.ka);
+        }
         const double c = (double)P.ka.R * (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
         // the problem's tiles in blocks of 8 tile rows (column-major inside a
         // block), dealt to the queues in contiguous runs
@@ -556,8 +590,9 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             const long in = lin - gi * per_group;
             const long tm = first_m + in % gsz, tn = in / gsz;
             const int q = (int)(lin * fmmk::kBatchQueues / ntiles);
-            qitems[g][q].push_back(BatchItem{pi, (int)(tm + tn * tiles_m)});
-            qcost[g][q].push_back(c);
+            const bool fast = pf >= 0 && fast_tile_inside(P.ka, tm, tn);
+            qitems[fast ? gf : g][q].push_back(BatchItem{fast ? pf : pi, (int)(tm + tn * tiles_m)});
+            qcost[fast ? gf : g][q].push_back(c);
         }
         if (P.presum) {
             for (const fmmhip_presum_seg_t* sg : {&P.segA, &P.segB}) {

@@ -971,6 +971,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     P->a_mn = a_mn;
     P->b_mn = b_mn;
     P->c_col = rsC == 1;
+    P->single_part = single_part;
     const int T = fmmk::tile_size(shape);
     const int wg_per_cu = shape == 2 ? 4 : 2;      // resident workgroups per CU (LDS and registers)
     ka.tiles_m = (int)((g.lm + T - 1) / T);

@@ -47,6 +47,7 @@ struct Prepared {
     fmmk::KArgs ka;              // kernel arguments (A/B are workspace offsets until bound)
     int shape;
     bool a_mn, b_mn, c_col;
+    bool single_part;            // one part per product and operand (classical, pre-summed)
     long nwg;
     int mode;                    // pack mode taken: 0 fused, 1 pre-summed, 2 products materialised
     bool untimed;                // never recorded by the launch timing
