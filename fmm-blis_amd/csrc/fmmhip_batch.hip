@@ -402,6 +402,7 @@ struct fmmhip_batch_s {
     bool planned = false;
     double est_ms = 0.0;
     long items[2] = {0, 0};     // work items per datatype (both tiles)
+    long fast_items[2] = {0, 0};  // of which on the fast kernel
 
     void release()
     {
@@ -603,6 +604,7 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     }
     // each queue costliest first, the queues of a group laid end to end
     b->items[0] = b->items[1] = 0;
+    b->fast_items[0] = b->fast_items[1] = 0;
     std::vector<int> qoff[kGroups];
     for (int g = 0; g < kGroups; g++) {
         qoff[g].push_back(0);
@@ -615,6 +617,7 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             qoff[g].push_back((int)items[g].size());
         }
         b->items[g % 2] += (long)items[g].size();
+        if (g / 2 == 2) b->fast_items[g % 2] += (long)items[g].size();
     }
     // block ranges of the summation segments: part-staged units are 256
     // elements along c (32 per block), entry-driven ones pairs (4096 per block)
@@ -717,6 +720,14 @@ extern "C" int fmmhip_batch_info(fmmhip_batch_t b, long* nprob, long* items_f64,
             if (P.presum) t += (P.ws_bytes + 255) & ~(size_t)255;
         *ws_bytes = t;
     }
+    return FMMHIP_SUCCESS;
+}
+
+extern "C" int fmmhip_batch_fast_items(fmmhip_batch_t b, long* fast_f64, long* fast_f32)
+{
+    if (!b || !b->planned) return FMMHIP_ERR_INVALID;
+    if (fast_f64) *fast_f64 = b->fast_items[1];
+    if (fast_f32) *fast_f32 = b->fast_items[0];
     return FMMHIP_SUCCESS;
 }
 

@@ -178,6 +178,7 @@ _SIGS = {
     "fmmhip_batch_choice": (_I, [_P, _L, ctypes.POINTER(_I)]),
     "fmmhip_batch_set_tile": (_I, [_P, _L, _I]),
     "fmmhip_batch_tile": (_I, [_P, _L, ctypes.POINTER(_I)]),
+    "fmmhip_batch_fast_items": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L)]),
     "fmmhip_tuner_choose_tile": (_I, [_I, _L, _L, _L, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                       ctypes.POINTER(_D)]),
     "fmmhip_tuner_choose": (_I, [_I, _L, _L, _L, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
@@ -459,6 +460,12 @@ class Batch:
     def set_tile(self, i, tile):
         """Force problem i's tile edge (64 or 128; 0 = the tuner's choice)."""
         check(lib().fmmhip_batch_set_tile(self.h, i, tile), "batch_set_tile")
+
+    def fast_items(self):
+        """(FP64, FP32) work items of the plan that run on the fast kernel."""
+        f64, f32 = _L(), _L()
+        check(lib().fmmhip_batch_fast_items(self.h, ctypes.byref(f64), ctypes.byref(f32)), "batch_fast_items")
+        return f64.value, f32.value
 
     def tile(self, i):
         """The planned tile edge of problem i."""

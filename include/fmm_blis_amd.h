@@ -420,6 +420,10 @@ int  fmmhip_batch_plan(fmmhip_batch_t b);
 int  fmmhip_batch_run(fmmhip_batch_t b, void* stream);
 int  fmmhip_batch_info(fmmhip_batch_t b, long* nprob, long* items_f64, long* items_f32, double* est_ms,
                        size_t* ws_bytes);
+/* Of the planned work items (problem, tile), how many run on the fast
+ * kernel (whole problems whose leaves it takes, and the interior tiles of
+ * problems on 128 x 128 tiles). */
+int  fmmhip_batch_fast_items(fmmhip_batch_t b, long* fast_f64, long* fast_f32);
 int  fmmhip_batch_choice(fmmhip_batch_t b, long i, int* cand);
 /* Forces problem i's tile edge (64 or 128; 0 = the tuner's choice);
  * invalidates a previous plan.  fmmhip_batch_tile reports the planned edge

@@ -294,6 +294,51 @@ def test_fast_kernel_fp32_equals_shape0(case):
             assert np.array_equal(outs[0].astype(np.float64), want), case
 
 
+# problems on 128 x 128 tiles whose leaves have interior tiles (run by the
+# fast kernel) and edge tiles (run by the general kernel): (dtype, m, n, k, cand)
+SPLIT = [("f8", 320, 448, 256, 0), ("f4", 448, 320, 192, 0), ("f8", 640, 576, 512, 1), ("f8", 1088, 960, 768, 2),
+         ("f4", 704, 832, 256, 1), ("f8", 392, 264, 128, 0)]
+
+
+def test_batch_interior_tiles_on_fast_kernel_equal_single_calls():
+    """Each problem's interior tiles go to the fast kernel and its edge tiles
+    to the general one; every C equals the same call alone bit for bit and,
+    on integer inputs, the oracle."""
+    for integer in (False, True):
+        b = F.Batch()
+        probs = []
+        for i, (dt, m, n, k, cand) in enumerate(SPLIT):
+            A, B, C = inputs(dt, m, n, k, 100 + i, integer)
+            alpha, beta = (1.0, 1.0) if integer else ALPHA_BETA[i % len(ALPHA_BETA)]
+            dA, dB, dC = dev(A), dev(B), dev(C)
+            b.add(m, n, k, alpha, dA, dB, beta, dC, cand=cand)
+            b.set_tile(i, 128)
+            probs.append((dt, m, n, k, cand, alpha, beta, A, B, C, dA, dB, dC))
+        b.plan()
+        b.run()
+        F.sync()
+        for i, (dt, m, n, k, cand, alpha, beta, A, B, C, dA, dB, dC) in enumerate(probs):
+            got = dC.to_numpy()
+            dC1 = dev(C)
+            alg = alg_for(cand)
+            alg.set_shape(0)
+            (F.dgemm_fmm if dt == "f8" else F.sgemm_fmm)(alg, m, n, k, alpha, dA, dB, beta, dC1)
+            F.sync()
+            alg.close()
+            assert np.array_equal(dC1.to_numpy(), got), (i, dt, m, n, k, cand, integer)
+            if integer:
+                A64, B64, C64 = (np.asfortranarray(X.astype(np.float64)) for X in (A, B, C))
+                want = O.dgemm(alpha, A64, B64, beta, C64.copy(order="F"))
+                assert np.array_equal(got.astype(np.float64), want), (i, dt, m, n, k, cand)
+        info = b.info()
+        fast64, fast32 = b.fast_items()
+        b.close()
+        say(f"split batch (integer={integer}): items f64 {info['items_f64']} ({fast64} fast), "
+            f"f32 {info['items_f32']} ({fast32} fast)")
+        # both kernels ran tiles of both datatypes
+        assert 0 < fast64 < info["items_f64"] and 0 < fast32 < info["items_f32"]
+
+
 def test_batch_graph_replays_with_eager_work_between():
     b, probs = build(integer=False)
     b.plan()
