@@ -147,6 +147,7 @@ def run_batch(args, F):
 
     tuned, plan_s = make(-1)
     info = tuned.info()
+    fast_items = tuned.fast_items()
     choices = {"f64": [0, 0, 0], "f32": [0, 0, 0]}
     tiles = {"f64": {"128": 0, "64": 0}, "f32": {"128": 0, "64": 0}}
     leaf64 = leaf32 = 0.0
@@ -204,6 +205,7 @@ def run_batch(args, F):
             "choices_classical_1level_2level": choices,
             "tile_edges": tiles,
             "items_f64": info["items_f64"], "items_f32": info["items_f32"],
+            "fast_kernel_items_f64_f32": list(fast_items),
             "workspace_bytes": info["workspace_bytes"], "tuner_est_ms": round(info["est_ms"], 3),
             "plan_s": round(plan_s, 3),
             "classical_batch_ms": round(cls_ms, 3) if cls_ms else None,
