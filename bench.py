@@ -135,6 +135,12 @@ def run_batch(args, F):
         for _ in range(warmup):
             run()
         F.sync()
+        # and at least 250 ms of warm-up: a batch of small problems replays in
+        # about a millisecond, and two replays leave the clocks ramping
+        t_end = time.perf_counter() + 0.25
+        while time.perf_counter() < t_end:
+            run()
+            F.sync()
         evs = [(F.Event(), F.Event()) for _ in range(steps)]
         t0 = time.perf_counter()
         for e0, e1 in evs:
