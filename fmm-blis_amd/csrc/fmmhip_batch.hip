@@ -2,10 +2,11 @@
 //
 // Many independent column-major GEMMs (FP64 and FP32) run as one captured
 // HIP graph: per datatype, one batched summation launch (the pre-summed
-// operand sums of every FMM problem) and one persistent MFMA launch whose
-// workgroups take (problem, tile) items from a device queue until it is
-// empty.  The algorithm, level and pack mode of every problem are chosen by
-// the tuner below.  No counterpart in the reference (its driver runs one
+// operand sums of every FMM problem), persistent MFMA launches (per tile
+// kernel) whose workgroups take (problem, tile, product group) items from
+// device queues until they are empty, and the W stages of the FMM problems
+// (their products materialised).  The algorithm, level and tile of every
+// problem are chosen by the tuner below.  No counterpart in the reference (its driver runs one
 // product at a time, frame/base/bli_strassen_ab.c:123-295).
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -23,14 +24,15 @@ namespace {
 
 // Whether a problem prepared for the general 128 x 128 kernel can run its
 // interior tiles on the fast kernel: the fast kernel's layouts, 16-byte
-// aligned pairs, no k-split, every part with the leaf's full k in whole
-// k-steps of the leaf type (d: FP64).
+// aligned pairs, no k-split, product groups only into the materialised
+// slots (no per-group copies of C), every part with the leaf's full k in
+// whole k-steps of the leaf type (d: FP64).
 bool fast_tiles_ok(const Prepared& P, int d)
 {
     const fmmk::KArgs& a = P.ka;
     const int bk = d ? fmmk::fast_bk<double>() : fmmk::fast_bk<float>();
-    return P.a_mn && !P.b_mn && P.c_col && a.va && a.vb && !a.ksplit && a.groups == 1 && a.lk % bk == 0 &&
-           a.lk >= bk && a.min_k == a.lk;
+    return P.a_mn && !P.b_mn && P.c_col && a.va && a.vb && !a.ksplit && (a.groups == 1 || P.wmat) &&
+           a.lk % bk == 0 && a.lk >= bk && a.min_k == a.lk;
 }
 
 // Whether tile (tm, tn) lies inside every part of A, B and C (the fast
@@ -49,9 +51,9 @@ namespace {
 // ---- tuner -------------------------------------------------------------------
 
 // Candidates: 0 classical, 1 one-level Strassen, 2 two-level Strassen (both
-// pre-summed), each on one of two tiles: 128 x 128 (launch shape 0) or
-// 64 x 64 (launch shape 2).  Cost model of one candidate and tile on one
-// problem:
+// pre-summed, products materialised), each on one of two tiles: 128 x 128
+// (launch shape 0) or 64 x 64 (launch shape 2).  Cost model of one
+// candidate and tile on one problem:
 //   t = F_leaf * pad / (P * eff * teff * nk/(nk + ovh)) * quant + bytes_sum / bw
 // F_leaf the leaf flops, pad the tile grid's padding (tiles * T^2 over the
 // leaf's lm * ln: the kernel runs whole tiles), P the MFMA peak of the
@@ -59,7 +61,8 @@ namespace {
 // tile's rate relative to it on whole tiles, nk the 16-wide k steps per
 // product, ovh the per-product prologue/epilogue cost in k steps, quant the
 // wave-quantisation loss of a lone launch without product split (1 inside a
-// batch), bytes_sum the summation stage's traffic and bw its bandwidth.
+// batch), bytes_sum the traffic of the summation and W stages and bw their
+// bandwidth.
 constexpr int kCands = 3;
 constexpr int kTiles = 2;                         // 0: 128 x 128, 1: 64 x 64
 const int kTileShape[kTiles] = {0, 2};            // their kernel launch shapes
@@ -67,7 +70,7 @@ const int kTileWgPerCu[kTiles] = {2, 4};          // resident workgroups per CU
 struct TunerParams {
     double eff[2][kCands];      // [is_double][cand]
     double ovh[2][kCands];
-    double bw;                  // summation bandwidth, bytes/s
+    double bw;                  // bandwidth of the summation and W stages, bytes/s
     double launch_ms;           // fixed cost per launch
     int cus;
     double teff[kTiles];        // rate of each tile's kernel relative to the 128 x 128 one (whole tiles)
@@ -76,11 +79,12 @@ struct TunerParams {
 std::mutex g_tuner_mu;
 TunerParams g_tp = {
     // defaults: fmmhip_tuner_calibrate on an MI355X (FP32 row, then FP64;
-    // 128 x 128 tiles on the fast kernel wherever it applies)
-    {{0.858, 0.845, 0.797}, {0.931, 0.914, 0.866}},
+    // 128 x 128 tiles on the fast kernel wherever it applies, FMM products
+    // materialised)
+    {{0.856, 0.859, 0.861}, {0.929, 0.925, 0.921}},
     {{4.0, 4.0, 4.0}, {4.0, 4.0, 4.0}},
-    3.97e12, 0.01, 256,
-    {1.0, 0.867}};               // 64 x 64 tiles against the fast kernel's 128 x 128 ones, classical FP64 16384^3
+    4.09e12, 0.01, 256,
+    {1.0, 0.88}};                // 64 x 64 tiles against the fast kernel's 128 x 128 ones, classical FP64 16384^3
 
 const double kPeak[2] = {157.3e12, 78.6e12};   // FP32, FP64 MFMA peaks (FLOP/s)
 
@@ -106,8 +110,11 @@ double est_ms(const TunerParams& tp, int is_double, int cand, long m, long n, lo
     }
     double t = fl / (kPeak[is_double] * eff);
     if (cand > 0) {
+        // summation (A and B read, R leaves of each written) and W stage
+        // (the products materialised: R slots read, C read and written)
         const double esz = is_double ? 8.0 : 4.0;
-        const double bytes = esz * ((double)m * k + c.R * lm * lk + (double)k * n + c.R * lk * ln);
+        const double bytes =
+            esz * ((double)m * k + c.R * lm * lk + (double)k * n + c.R * lk * ln + c.R * lm * ln + 2.0 * m * n);
         t += bytes / tp.bw;
     }
     return t * 1e3 + (lone ? tp.launch_ms * (cand > 0 ? 3 : 1) : 0.0);
@@ -118,10 +125,12 @@ double est_ms(const TunerParams& tp, int is_double, int cand, long m, long n, lo
 // (workspace, a summation launch, long work items) than a wrong classical one.
 // Where the candidate's leaves are whole 128 x 128 tiles and whole fast-kernel
 // k-steps (every tile on the fast kernel, the regime the calibration times)
-// the model was within 2% of the measured batch times (4 x 12288^3 FP64:
-// 204 / 189 / 180 ms estimated against 202 / 191 / 184 ms for classical, one-
-// and two-level Strassen) and the factor is 0.97; elsewhere edge tiles run
-// the general kernel, which the model does not price, and the factor is 0.95.
+// the model came within 4% of the measured batches, on the cautious side for
+// FMM (FP64, classical / two-level Strassen: 4 x 12288^3 204 / 176 ms
+// estimated, 203 / 170 measured; 8 x 8192^3 121 / 110, 121 / 106;
+// 16 x 6144^3 103 / 97, 103 / 94) and the factor is 0.97; elsewhere edge
+// tiles run the general kernel, which the model does not price, and the
+// factor is 0.95.
 constexpr double kFmmMargin = 0.95, kFmmMarginWhole = 0.97;
 
 bool whole_fast_leaves(int is_double, int cand, long m, long n, long k)
@@ -223,7 +232,9 @@ extern "C" int fmmhip_tuner_estimate(int is_double, int cand, long m, long n, lo
 // Times each candidate at 16384^3 (leaves of 16384, 8192 and 4096: 16384,
 // 4096 and 1024 tiles of 128x128, whole waves of the device's workgroup
 // slots) with the library's launch timing, and refits its sustained
-// efficiency (at the fixed prologue cost ovh) and the summation bandwidth.
+// efficiency (at the fixed prologue cost ovh) and the bandwidth of the
+// summation and W stages (FMM candidates run with their products
+// materialised, as in the batch and the server).
 // Large, device-filling launches give the steady-state efficiency that a
 // batch (no tails) and a busy server see; smaller calibration sizes folded
 // the launch tail into the classical figure and made the model favour FMM
@@ -266,8 +277,8 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
         for (int cand = 0; cand < kCands && !rc; cand++) {
             fmmhip_alg_t alg = cand_alg(cand);
             const int saved_pack = alg->pack, saved_split = alg->split;
-            alg->pack = cand > 0 ? 1 : 0;
-            alg->split = 1;             // as the batch and the server run them
+            alg->pack = cand > 0 ? 2 : 0;   // as the batch and the server run them
+            alg->split = cand > 0 ? -1 : 1;
             const long nn = nmax;
             for (int rep = 0; rep < 3 && !rc; rep++) {
                 if (rep == 1) fmmhip_timing_enable(1);
@@ -295,7 +306,10 @@ extern "C" int fmmhip_tuner_calibrate(void* stream)
             if (eff > 0.98) eff = 0.98;
             if (eff > 0.05) tp.eff[dbl][cand] = eff;
             if (cand > 0 && pms > 0) {
-                const double bytes = (double)esz * (2.0 * nn * nn + 2.0 * c.R * leaf * leaf);
+                // the timed stages: the summation (A and B read, R leaves of
+                // each written) and the W stage (R slots read, C read and written)
+                const double bytes = (double)esz * (2.0 * nn * nn + 2.0 * c.R * leaf * leaf + c.R * leaf * leaf +
+                                                    2.0 * nn * nn);
                 bw_sum += bytes / (pms / nl * 1e-3);
                 bw_n += 1;
             }
@@ -510,9 +524,17 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     const size_t np = b->probs.size();
     b->prep.assign(np, Prepared());
     b->est_ms = 0.0;
-    // 1. choices and preparation (descriptors uploaded on the batch stream)
+    // 1. choices and preparation (descriptors uploaded on the batch stream).
+    //    FMM problems materialise their products (each product stored once
+    //    into its slot by product groups, then the W stage), as single calls
+    //    do where the slots fit; pre-summed with the W epilogue fused when the
+    //    batch's workspace would pass the bound.  Both give the same C.
     size_t ws_total = 0;
     std::vector<size_t> ws_off(np, 0);
+    int pack_fmm = 2;
+  prepare_all:
+    ws_total = 0;
+    b->est_ms = 0.0;
     for (size_t i = 0; i < np && !rc; i++) {
         fmmhip_batch_s::Prob& p = b->probs[i];
         double est = 0.0;
@@ -536,8 +558,10 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             }
         }
         b->est_ms += est;
+        b->prep[i] = Prepared();
         rc = prepare_call(b->algs[p.tile][p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1, p.lda,
-                          p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? 1 : 0, 1, b->stream, &b->prep[i]);
+                          p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? pack_fmm : 0, p.cand > 0 ? -1 : 1,
+                          b->stream, &b->prep[i]);
         // the batched kernel is instantiated for column-major operands only
         if (!rc && !b->prep[i].nothing && !b->prep[i].scale_only &&
             !(b->prep[i].a_mn && !b->prep[i].b_mn && b->prep[i].c_col))
@@ -548,6 +572,10 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         }
     }
     if (rc) return rc;
+    if (pack_fmm == 2 && !((double)ws_total < fmmhip_ws_bound())) {
+        pack_fmm = 1;
+        goto prepare_all;
+    }
     if (ws_total) {
         if (hipMalloc(&b->ws, ws_total) != hipSuccess) {
             (void)hipGetLastError();
@@ -594,21 +622,28 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             pf = (int)ka[gf].size();
             ka[gf].push_back(P.ka);
         }
-        const double c = (double)P.ka.R * (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
-        // the problem's tiles in blocks of 8 tile rows (column-major inside a
-        // block), dealt to the queues in contiguous runs
+        const double ksteps = (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
+        // per product group (materialised problems; else one group of all
+        // products), the problem's tiles in blocks of 8 tile rows (column-
+        // major inside a block), dealt to the queues in contiguous runs; an
+        // item's tile index carries its group: tile + group * tiles
         const int tiles_m = P.ka.tiles_m, tiles_n = P.ka.tiles_n;
         const long ntiles = (long)tiles_m * tiles_n;
-        for (long lin = 0; lin < ntiles; lin++) {
-            const long per_group = 8L * tiles_n;
-            const long gi = lin / per_group, first_m = gi * 8;
-            const long gsz = std::min<long>(tiles_m - first_m, 8);
-            const long in = lin - gi * per_group;
-            const long tm = first_m + in % gsz, tn = in / gsz;
-            const int q = (int)(lin * fmmk::kBatchQueues / ntiles);
-            const bool fast = pf >= 0 && fast_tile_inside(P.ka, tm, tn);
-            qitems[fast ? gf : g][q].push_back(BatchItem{fast ? pf : pi, (int)(tm + tn * tiles_m)});
-            qcost[fast ? gf : g][q].push_back(c);
+        const int G = P.ka.groups, R = P.ka.R;
+        for (int gr = 0; gr < G; gr++) {
+            const double c = (double)((long)(gr + 1) * R / G - (long)gr * R / G) * ksteps;
+            for (long lin = 0; lin < ntiles; lin++) {
+                const long per_group = 8L * tiles_n;
+                const long gi = lin / per_group, first_m = gi * 8;
+                const long gsz = std::min<long>(tiles_m - first_m, 8);
+                const long in = lin - gi * per_group;
+                const long tm = first_m + in % gsz, tn = in / gsz;
+                const int q = (int)(lin * fmmk::kBatchQueues / ntiles);
+                const bool fast = pf >= 0 && fast_tile_inside(P.ka, tm, tn);
+                qitems[fast ? gf : g][q].push_back(
+                    BatchItem{fast ? pf : pi, (int)(tm + tn * tiles_m + (long)gr * ntiles)});
+                qcost[fast ? gf : g][q].push_back(c);
+            }
         }
         if (P.presum) {
             for (const fmmhip_presum_seg_t* sg : {&P.segA, &P.segB}) {
@@ -700,6 +735,13 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
             else
                 rc = fmmk_launch_batch_s32((const KArgs*)(dv + o_ka[g]), (const BatchItem*)(dv + o_it[g]), qo, heads,
                                            vec[g], shape, nwg, b->stream);
+        }
+        // the W stages of the datatype's materialised problems
+        for (size_t i = 0; i < np && !rc; i++) {
+            const Prepared& P = b->prep[i];
+            if (b->probs[i].dbl != d || P.nothing || P.scale_only || !P.wmat) continue;
+            rc = fmmhip_wsum(d, &P.wsum, b->stream);
+            if (!rc && P.w2) rc = fmmhip_wsum(d, &P.wsum2, b->stream);
         }
     }
     hipGraph_t g = nullptr;

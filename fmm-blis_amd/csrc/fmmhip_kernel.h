@@ -864,7 +864,7 @@ inline bool fast_eligible(const KArgs& ka, bool a_mn, bool b_mn, bool c_col)
 }
 
 // Batched persistent kernel: a fixed grid of workgroups takes work items
-// (problem, tile) from device queues until they are empty.  There is one
+// (problem, tile, product group) from device queues until they are empty.  There is one
 // queue per XCD (workgroup b of a grid that is resident at once runs on XCD
 // b mod 8): the host deals each problem's tiles to the queues in contiguous
 // blocks of its tile grid, so the workgroups of one XCD share A and B panels
@@ -874,7 +874,7 @@ inline bool fast_eligible(const KArgs& ka, bool a_mn, bool b_mn, bool c_col)
 constexpr int kBatchQueues = 8;
 struct BatchItem {
     int problem;
-    int tile;                    // linear tile index in the problem's leaf grid
+    int tile;                    // linear tile index in the problem's leaf grid, + product group x tiles
 };
 
 // The next work item of the workgroup (uniform), or -1 when every queue
@@ -911,14 +911,16 @@ void fmm_batch_kernel(const KArgs* __restrict__ probs, const BatchItem* __restri
         if (w < 0) break;
         const BatchItem it = items[w];
         const KArgs a = probs[it.problem];
-        const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
-        gemm_tile<TI, TL, TO, SHAPE, VEC, true, false, true>(a, tm, tn, 0, sA, sB);
+        const int nt = a.tiles_m * a.tiles_n, grp = it.tile / nt, t = it.tile - grp * nt;
+        const int tm = t % a.tiles_m, tn = t / a.tiles_m;
+        gemm_tile<TI, TL, TO, SHAPE, VEC, true, false, true>(a, tm, tn, grp, sA, sB);
         __syncthreads();                       // LDS free before the next item's prologue
     }
 }
 
-// The batched form of the fast kernel (problems the host found eligible;
-// one product group per tile).
+// The batched form of the fast kernel (problems the host found eligible,
+// and the interior tiles of the others; an item is one tile of one product
+// group).
 template <typename TI, typename TL, typename TO>
 __global__ __attribute__((amdgpu_flat_work_group_size(kFastNT, kFastNT), amdgpu_waves_per_eu(2, 2)))
 void fmm_batch_fast_kernel(const KArgs* __restrict__ probs, const BatchItem* __restrict__ items,
@@ -933,8 +935,9 @@ void fmm_batch_fast_kernel(const KArgs* __restrict__ probs, const BatchItem* __r
         if (w < 0) break;
         const BatchItem it = items[w];
         const KArgs a = probs[it.problem];
-        const int tm = it.tile % a.tiles_m, tn = it.tile / a.tiles_m;
-        fast_tile<TI, TL, TO>(a, tm, tn, 0, sA, sB);
+        const int nt = a.tiles_m * a.tiles_n, grp = it.tile / nt, t = it.tile - grp * nt;
+        const int tm = t % a.tiles_m, tn = t / a.tiles_m;
+        fast_tile<TI, TL, TO>(a, tm, tn, grp, sA, sB);
         __syncthreads();                       // LDS free before the next item's prologue
     }
 }
