@@ -95,11 +95,30 @@ int upload_list(DevList& l, int R, hipStream_t stream)
     return FMMHIP_SUCCESS;
 }
 
+// Whether the pre-summed leaves and the product slots of geometry g are laid
+// out for whole 128 x 128 tiles: column-major A, B and C whose leaf k is a
+// whole number of k-steps of either fast kernel (32) and whose leaf m or n
+// is not a multiple of 128.  A's leaves then take 128-aligned columns, the
+// slots 128-aligned rows and columns, and B's leaves a tail after the last
+// one, so that a kernel running whole tiles (the fast kernel) reads and
+// writes inside the workspace.  The padding rows of A and columns of B it
+// reads (never written; B's are the next leaf's first columns) meet only
+// the padding rows and columns of the slots, which nothing reads; k is not
+// padded, so every product of the valid region is the unpadded one.
+bool tile_padded(const fmm_geom_t& g)
+{
+    return g.rsA == 1 && g.rsB == 1 && g.rsC == 1 && g.lk > 0 && g.lk % 32 == 0 &&
+           (g.lm % fmmk::BM != 0 || g.ln % fmmk::BN != 0);
+}
+long up_tile(long x) { return (x + fmmk::BM - 1) / fmmk::BM * fmmk::BM; }
+
 // Workspace leaf layout of the pre-summed mode for one operand: R leaves of
-// Lc x Ls (contiguous dim c), leading dimension ldo = Lc rounded up to even.
+// Lc x Ls (contiguous dim c), leading dimension ldo = Lc rounded up to even
+// (to 128 for A's leaves of a tile-padded geometry), then `tail` elements.
 struct WsLayout {
     int Lc, Ls, ldo;
-    size_t elems(int R) const { return (size_t)ldo * Ls * R; }
+    long tail;
+    size_t elems(int R) const { return (size_t)ldo * Ls * R + (size_t)tail; }
 };
 
 WsLayout ws_layout_a(const fmm_geom_t& g, bool a_mn)
@@ -107,7 +126,8 @@ WsLayout ws_layout_a(const fmm_geom_t& g, bool a_mn)
     WsLayout w;
     w.Lc = (int)(a_mn ? g.lm : g.lk);
     w.Ls = (int)(a_mn ? g.lk : g.lm);
-    w.ldo = (w.Lc + 1) & ~1;
+    w.ldo = a_mn && tile_padded(g) ? (int)up_tile(w.Lc) : (w.Lc + 1) & ~1;
+    w.tail = 0;
     return w;
 }
 
@@ -117,6 +137,7 @@ WsLayout ws_layout_b(const fmm_geom_t& g, bool b_mn)
     w.Lc = (int)(b_mn ? g.ln : g.lk);
     w.Ls = (int)(b_mn ? g.lk : g.ln);
     w.ldo = (w.Lc + 1) & ~1;
+    w.tail = !b_mn && tile_padded(g) ? (up_tile(g.ln) - g.ln) * w.ldo : 0;
     return w;
 }
 
@@ -191,8 +212,9 @@ SlotLayout slot_layout(const fmm_geom_t& g)
     sl.swap = (g.csC == 1 && g.rsC != 1) ? 1 : 0;
     sl.Lc = (int)(sl.swap ? g.ln : g.lm);
     sl.Ls = (int)(sl.swap ? g.lm : g.ln);
-    sl.ldm = sl.Lc;
-    sl.slot = sl.ldm * sl.Ls;
+    const bool pad = !sl.swap && tile_padded(g);
+    sl.ldm = pad ? up_tile(sl.Lc) : sl.Lc;
+    sl.slot = sl.ldm * (pad ? up_tile(sl.Ls) : sl.Ls);
     return sl;
 }
 
@@ -1000,6 +1022,11 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.rsC = sl.swap ? sl.ldm : 1;
         ka.csC = sl.swap ? 1 : sl.ldm;
         ka.min_cm = (int)g.lm; ka.min_cn = (int)g.ln;
+        if (tile_padded(g) && a_mn && !b_mn && rsC == 1 && T == fmmk::BM) {
+            // whole tiles over the padded leaves and slots (same tile grid)
+            ka.lm = ka.min_m = ka.min_cm = (int)up_tile(g.lm);
+            ka.ln = ka.min_n = ka.min_cn = (int)up_tile(g.ln);
+        }
         ka.alpha = 1.0;
         ka.beta = 0.0;
         ka.ptr = dv.mat_ptr;
