@@ -310,8 +310,14 @@ __global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ 
                     default: break;
                     }
                 }
-                out[(long)r * sg.pstride] = (T)v0;
-                if (two) out[(long)r * sg.pstride + 1] = (T)v1;
+                // one store of the pair: S, s_off, pstride and ldo are even
+                // element counts from a 256-byte aligned base, and c0 is even
+                if (two) {
+                    typedef T pair2 __attribute__((ext_vector_type(2)));
+                    *(pair2*)(out + (long)r * sg.pstride) = pair2{(T)v0, (T)v1};
+                } else {
+                    out[(long)r * sg.pstride] = (T)v0;
+                }
             }
         }
     }
@@ -353,6 +359,9 @@ extern "C" int fmmhip_presum_stage(int is_double, const void* X, long ld, void* 
                                    const fmmhip_stage_seg_t* segs, int nseg, void* stream)
 {
     if (nseg <= 0 || Lc <= 0 || Ls <= 0) return FMMHIP_SUCCESS;
+    // the kernel stores element pairs as one vector (the segments' offsets
+    // and strides are even element counts by construction)
+    if ((ldo & 1) || ((uintptr_t)S & (2 * (is_double ? sizeof(double) : sizeof(float)) - 1))) return FMMHIP_ERR_INVALID;
     constexpr int TB = 256;
     // about 65536 blocks in all (grid-strided), at most one per unit of a segment
     const long units = (long)((Lc + 2 * TB - 1) / (2 * TB)) * Ls;
