@@ -625,8 +625,10 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
         const double ksteps = (double)((P.ka.lk + fmmk::BK - 1) / fmmk::BK);
         // per product group (materialised problems; else one group of all
         // products), the problem's tiles in blocks of 8 tile rows (column-
-        // major inside a block), dealt to the queues in contiguous runs; an
-        // item's tile index carries its group: tile + group * tiles
+        // major inside a block), dealt to the queues in contiguous runs per
+        // group, or, where the launch would deal whole groups to the XCDs
+        // (xcd_groups), the (group, tile) sequence group-major in contiguous
+        // runs; an item's tile index carries its group: tile + group * tiles
         const int tiles_m = P.ka.tiles_m, tiles_n = P.ka.tiles_n;
         const long ntiles = (long)tiles_m * tiles_n;
         const int G = P.ka.groups, R = P.ka.R;
@@ -638,7 +640,8 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
                 const long gsz = std::min<long>(tiles_m - first_m, 8);
                 const long in = lin - gi * per_group;
                 const long tm = first_m + in % gsz, tn = in / gsz;
-                const int q = (int)(lin * fmmk::kBatchQueues / ntiles);
+                const int q = P.ka.xcd_groups ? (int)(((long)gr * ntiles + lin) * fmmk::kBatchQueues / ((long)G * ntiles))
+                                              : (int)(lin * fmmk::kBatchQueues / ntiles);
                 const bool fast = pf >= 0 && fast_tile_inside(P.ka, tm, tn);
                 qitems[fast ? gf : g][q].push_back(
                     BatchItem{fast ? pf : pi, (int)(tm + tn * tiles_m + (long)gr * ntiles)});

@@ -770,6 +770,14 @@ int split_groups(int requested, int max_groups, long tiles, double leaf_flops, d
 // G = R the blocks of one product run together and share its S_r row and
 // T_r column panels in L2, as a classical GEMM's tiles share A and B (L2
 // hit rate 0.50 with one group, 0.77 with 49, at m = n = k = 16384, 222 L2).
+// Whether a launch of `groups` groups of `tiles` tiles deals whole groups to
+// the XCDs: several groups, each of no more tiles than an XCD holds
+// workgroups at once (its CUs' share of the resident slots).
+int xcd_groups(int groups, long tiles, int wg_per_cu, int cus)
+{
+    return groups > 1 && tiles <= (long)wg_per_cu * cus / 8 ? 1 : 0;
+}
+
 int mat_groups(int R, long tiles, int slots)
 {
     int best = 1;
@@ -1035,6 +1043,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.groups = split > 0 ? std::min(split, alg->R) : mat_groups(alg->R, tiles, wg_per_cu * cus);
         ka.ksplit = false;
         ka.gbit = 0;
+        ka.xcd_groups = xcd_groups(ka.groups, tiles, wg_per_cu, cus);
         ka.tmp = ka.C;           // group g's Cb: the slots (no per-group copy)
         ka.tmp_elems = 0;
         fmmhip_wsum_t& w = P->wsum;
@@ -1092,6 +1101,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
                              kind == Kind::D64, 0.0, wg_per_cu);
     ka.ksplit = kspl && ka.groups > 1;
     ka.gbit = ka.groups == 8 ? 3 : ka.groups == 4 ? 2 : ka.groups == 2 ? 1 : 0;
+    ka.xcd_groups = xcd_groups(ka.groups, tiles, wg_per_cu, cus);
     ka.tmp = nullptr;
     ka.tmp_elems = 0;
     if (ka.groups > 1) {

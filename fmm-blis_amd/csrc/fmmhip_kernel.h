@@ -110,8 +110,9 @@ struct KArgs {
     // taking products [g*R/groups, (g+1)*R/groups) and scattering into its
     // own temporary tmp + g*tmp_elems (C's strides; zeroed beforehand, reduced
     // into C afterwards), first writers from bit gbit of the entries' flags.
-    // ksplit: the groups split the k-steps instead (R == 1, classical)
-    int groups, gbit, ksplit;
+    // ksplit: the groups split the k-steps instead (R == 1, classical).
+    // xcd_groups: the launch deals whole groups to the XCDs (group_tile_of)
+    int groups, gbit, ksplit, xcd_groups;
     void* tmp;
     long tmp_elems;
     const int* ptr;              // u, v, w CSR pointers, 3*(R+1)
@@ -313,11 +314,8 @@ struct Stage {
 // contiguous run of tiles, then the run is walked in groups of 8 tile rows
 // (column-major inside a group) so that concurrently running workgroups on
 // one XCD share A row panels and B column panels in that XCD's L2.
-__device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& tm, int& tn)
+__device__ __forceinline__ void tile_walk(int lin, int tiles_m, int tiles_n, int& tm, int& tn)
 {
-    const int nwg = tiles_m * tiles_n;
-    int lin = bid;
-    if ((nwg & 7) == 0) lin = (bid & 7) * (nwg >> 3) + (bid >> 3);
     const int GROUP = 8;
     const int per_group = GROUP * tiles_n;
     const int g = lin / per_group;
@@ -326,6 +324,40 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     const int in = lin - g * per_group;
     tm = first_m + in % gsz;
     tn = in / gsz;
+}
+
+__device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& tm, int& tn)
+{
+    const int nwg = tiles_m * tiles_n;
+    int lin = bid;
+    if ((nwg & 7) == 0) lin = (bid & 7) * (nwg >> 3) + (bid >> 3);
+    tile_walk(lin, tiles_m, tiles_n, tm, tn);
+}
+
+// The order over a grid of `groups` product (or k-step) groups of tiles.
+// By default block g * tiles + t is tile t (tile_of's order) of group g, so
+// the XCDs share each product's tiles.  With xcd, the ids are regrouped over
+// the whole grid so that each XCD receives a contiguous run of (group,
+// tile) pairs, group-major: whole groups per XCD, whose workgroups share
+// their products' leaves in that XCD's L2.  The host asks for it where a
+// product has no more tiles than an XCD holds workgroups at once (444 L2 at
+// 13824^3, 49 tiles per product: spread over the XCDs, every product was
+// read into every XCD's L2, L2 hit 0.17); a product of many tiles walked by
+// one XCD re-reads its panels more often than eight XCDs sharing it.
+__device__ __forceinline__ void group_tile_of(int bid, int groups, bool xcd, int tiles_m, int tiles_n, int& grp,
+                                              int& tm, int& tn)
+{
+    const int tiles = tiles_m * tiles_n;
+    if (groups == 1 || !xcd) {
+        grp = bid / tiles;
+        tile_of(bid - grp * tiles, tiles_m, tiles_n, tm, tn);
+        return;
+    }
+    const int nwg = groups * tiles;
+    const int x = bid & 7, q = nwg >> 3, r = nwg & 7;
+    const int lin = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (bid >> 3);
+    grp = lin / tiles;
+    tile_walk(lin - grp * tiles, tiles_m, tiles_n, tm, tn);
 }
 
 // One 16-column block (column block j of the wave's accumulators) of the W
@@ -592,11 +624,8 @@ void fmm_gemm_kernel(KArgs a)
 {
     __shared__ TL sA[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
     __shared__ TL sB[2][lds_tile<TL, Cfg<SHAPE>::TILE>()];
-    // blocks [g*tiles, (g+1)*tiles) are product group g
-    const int tiles = a.tiles_m * a.tiles_n;
-    const int grp = (int)blockIdx.x / tiles;
-    int tm, tn;
-    tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
+    int grp, tm, tn;
+    group_tile_of((int)blockIdx.x, a.groups, a.xcd_groups != 0, a.tiles_m, a.tiles_n, grp, tm, tn);
     gemm_tile<TI, TL, TO, SHAPE, VEC, A_MN, B_MN, C_COL, KSPL>(a, tm, tn, grp, sA, sB);
 }
 
@@ -843,11 +872,8 @@ void fmm_fast_kernel(KArgs a)
 {
     __shared__ TL sA[2][fast_lds<TL>()];
     __shared__ TL sB[2][fast_lds<TL>()];
-    // blocks [g*tiles, (g+1)*tiles) are product group g
-    const int tiles = a.tiles_m * a.tiles_n;
-    const int grp = (int)blockIdx.x / tiles;
-    int tm, tn;
-    tile_of((int)blockIdx.x - grp * tiles, a.tiles_m, a.tiles_n, tm, tn);
+    int grp, tm, tn;
+    group_tile_of((int)blockIdx.x, a.groups, a.xcd_groups != 0, a.tiles_m, a.tiles_n, grp, tm, tn);
     fast_tile<TI, TL, TO>(a, tm, tn, grp, sA, sB);
 }
 
