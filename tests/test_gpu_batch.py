@@ -294,6 +294,52 @@ def test_fast_kernel_fp32_equals_shape0(case):
             assert np.array_equal(outs[0].astype(np.float64), want), case
 
 
+# (dtype or leaf, m, n, k, levels): products materialised with leaf m or n
+# not a multiple of 128 and leaf k a multiple of 32, so the leaves and slots
+# are padded to whole tiles; the few tiles per product also make the launch
+# deal whole product groups to the XCDs
+PADDED = [("f8", 600, 520, 512, 1), ("f8", 1000, 1000, 1024, 2), ("f8/f32", 600, 520, 512, 1),
+          ("f4", 700, 900, 1024, 2), ("f8", 1400, 300, 768, 1)]
+
+
+@pytest.mark.parametrize("case", PADDED, ids=lambda c: "-".join(map(str, c)).replace("/", "_"))
+def test_padded_leaves_on_fast_kernel(case):
+    """The materialised mode over tile-padded leaves takes the fast kernel
+    (automatic shape) and gives the same C, bit for bit, as the general
+    kernel over the same leaves and as the pre-summed mode (unpadded kernel
+    extents, W epilogue fused); integer inputs equal the oracle."""
+    kind, m, n, k, levels = case
+    i = PADDED.index(case)
+    dt = "f4" if kind == "f4" else "f8"
+    for integer in (False, True):
+        A, B, C = inputs(dt, m, n, k, 140 + i, integer)
+        alpha, beta = (0.7, -1.3) if not integer else (2.0, -1.0)
+        dA, dB = dev(A), dev(B)
+        outs, shapes = [], []
+        for pack, shape, split in ((2, -1, -1), (2, 0, -1), (1, 0, 1)):
+            alg = F.Alg([F.strassen()] * levels)
+            alg.set_pack(pack)
+            alg.set_shape(shape)
+            alg.set_split(split)
+            dC = dev(C)
+            if kind == "f4":
+                F.sgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC)
+            elif kind == "f8":
+                F.dgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC)
+            else:
+                F.dgemm_fmm(alg, m, n, k, alpha, dA, dB, beta, dC, leaf=F.LEAF_F32)
+            F.sync()
+            outs.append(dC.to_numpy())
+            shapes.append(alg.last_shape())
+            alg.close()
+        assert shapes[:2] == [3, 0], (case, shapes)
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2]), (case, integer)
+        if integer:
+            A64, B64, C64 = (np.asfortranarray(X.astype(np.float64)) for X in (A, B, C))
+            want = O.dgemm(alpha, A64, B64, beta, C64.copy(order="F"))
+            assert np.array_equal(outs[0].astype(np.float64), want), case
+
+
 # problems on 128 x 128 tiles whose leaves have interior tiles (run by the
 # fast kernel) and edge tiles (run by the general kernel): (dtype, m, n, k, cand)
 SPLIT = [("f8", 320, 448, 256, 0), ("f4", 448, 320, 192, 0), ("f8", 640, 576, 512, 1), ("f8", 1088, 960, 768, 2),
