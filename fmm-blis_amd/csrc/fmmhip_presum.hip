@@ -338,9 +338,12 @@ int launch_parts(const fmmhip_presum_seg_t* g, hipStream_t st)
 template <typename T>
 int launch_parts_typed(const fmmhip_presum_seg_t* g, hipStream_t st)
 {
-    // LDS per block = NPMAX * TB * 8 bytes: 32 KB, 64 KB, 64 KB, 128 KB
+    // LDS per block = NPMAX * TB * 8 bytes: 32 KB, 64 KB, 40 KB, 64 KB,
+    // 128 KB (36 parts, 2-level <2,3,2> and <3,2,2>: the 40-part form holds
+    // four blocks per CU where the 64-part one held two)
     if (g->np <= 16) return launch_parts<T, 16, 256>(g, st);
     if (g->np <= 32) return launch_parts<T, 32, 256>(g, st);
+    if (g->np <= 40) return launch_parts<T, 40, 128>(g, st);
     if (g->np <= 64) return launch_parts<T, 64, 128>(g, st);
     if (g->np <= 256) return launch_parts<T, 256, 64>(g, st);
     return FMMHIP_ERR_UNSUPPORTED;

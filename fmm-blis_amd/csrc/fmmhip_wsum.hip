@@ -16,7 +16,7 @@
 //
 // Two passes: per part (wsum_kernel: each part reads its products, so a
 // product is read once per W entry), or all parts of a group at once
-// (wsum_cols_kernel, groups of at most 16 parts over at most 64 slots: a
+// (wsum_cols_kernel, groups of at most 16 parts over at most 128 slots: a
 // thread keeps every part's value at its element in registers and reads
 // each slot once, applying it to the parts that take it; 222 L2, one
 // group: 49 slot reads and 32 C accesses per element instead of 144 and
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void wsum_kernel(fmmhip_wsum_t w, int cbl
     }
 }
 
-constexpr int kColsMaxR = 64;    // products of the all-parts pass
+constexpr int kColsMaxR = 128;   // products of the all-parts pass
 constexpr int kColsMaxP = 16;    // parts of the all-parts pass
 
 // All parts of one group at once.  blockIdx.y: the group; block item bb:
