@@ -184,6 +184,9 @@ int fmmhip_presum_s(const float* X, long ld, const int* ptr, const fmm_entry_t* 
 /* The same sums from a part table: every part element is read once and all
  * R sums are formed from it (identical arithmetic, fewer HBM reads). */
 int fmmhip_presum_seg(int is_double, const fmmhip_presum_seg_t* seg, void* stream);
+/* FP64 operand, FP32 leaves (S of type float): each sum formed in FP64 and
+ * rounded once to FP32. */
+int fmmhip_presum_seg_d2s(const fmmhip_presum_seg_t* seg, void* stream);
 
 #ifdef __cplusplus
 }

@@ -902,6 +902,16 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     g.rsA = rsA; g.csA = csA; g.rsB = rsB; g.csB = csB; g.rsC = rsC; g.csC = csC;
     const size_t esz = kind == Kind::S32 ? 4 : 8;
     const bool multi = alg->max_nnz_u > 1 || alg->max_nnz_v > 1;
+    // FP32-leaf kind with column-major operands whose sums run the one-pass
+    // part-staged kernel: the pre-summed leaves may be FP32 (each FP64 sum
+    // rounded once, as the kernel rounds it while staging an FP64 leaf), so
+    // the kernel and the summation move half the leaf bytes; taken below
+    // where the call runs on the fast kernel (the general kernel measured
+    // slower over FP32 leaves: 222 L2 at 10000^3 17.7 -> 19.5 ms).  The
+    // workspace keeps its FP64 size, so the pack mode does not depend on it.
+    const bool f32_possible = kind == Kind::D_F32LEAF && a_mn && !b_mn && rsC == 1 &&
+                              !kron_operand(alg, true) && !kron_operand(alg, false) &&
+                              (long)alg->mt * alg->kt <= kPartStagedMax && (long)alg->kt * alg->nt <= kPartStagedMax;
 
     // pack mode: pre-summed when asked, or (automatic) for multi-part
     // algorithms when the workspace fits in half of the free device memory
@@ -1095,6 +1105,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
         if (tiles <= 0) P->nothing = true;
         pick_fast(P, alg, kind, single_part);
+        P->f32_leaves = f32_possible && P->shape == 3;
         return FMMHIP_SUCCESS;
     }
     ka.groups = split_groups(split, alg->R == 1 ? (kspl ? (int)std::min(ksteps / 8, 8L) : 1) : alg->R, tiles,
@@ -1119,6 +1130,7 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
     if (P->nwg > INT32_MAX) return FMMHIP_ERR_UNSUPPORTED;
     if (tiles <= 0) P->nothing = true;
     pick_fast(P, alg, kind, single_part);
+    P->f32_leaves = presum && f32_possible && P->shape == 3;
     return FMMHIP_SUCCESS;
 }
 
@@ -1194,7 +1206,8 @@ int launch_prepared(const Prepared* P, int stages)
                 if (!r) r = fmmhip_presum_stage(dbl, P->x1, sg.ldo, sg.S, sg.swap, sg.Lc, sg.Ls, sg.ldo, s2, n2, st);
                 return r;
             }
-            if (sg.parts && sg.np <= kPartStagedMax) return fmmhip_presum_seg(dbl, &sg, st);
+            if (sg.parts && sg.np <= kPartStagedMax)
+                return P->f32_leaves ? fmmhip_presum_seg_d2s(&sg, st) : fmmhip_presum_seg(dbl, &sg, st);
             if (P->kind == Kind::S32)
                 return fmmhip_presum_s((const float*)sg.X, sg.ld, sg.ptr, sg.ent, sg.R, sg.swap, sg.Lc, sg.Ls, sg.ldo,
                                        (float*)sg.S, st);
@@ -1211,7 +1224,8 @@ int launch_prepared(const Prepared* P, int stages)
     switch (P->kind) {
     case Kind::D64: rc = fmmk_launch_d64(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
     case Kind::D_F32LEAF:
-        rc = fmmk_launch_d_f32leaf(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st);
+        rc = P->f32_leaves ? fmmk_launch_f32leaves_d(P->ka, P->shape, (int)P->nwg, st)
+                           : fmmk_launch_d_f32leaf(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st);
         break;
     case Kind::S32: rc = fmmk_launch_s32(P->ka, P->shape, P->a_mn, P->b_mn, P->c_col, (int)P->nwg, st); break;
     }

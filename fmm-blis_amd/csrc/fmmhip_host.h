@@ -62,6 +62,7 @@ struct Prepared {
     size_t m_off;                // slots: workspace offset
     fmmhip_wsum_t wsum;          // the W stage's arguments (slots bound with the workspace)
     bool w2;                     // two-stage W: wsum into the Y leaves, then wsum2 into C
+    bool f32_leaves;             // FP32-leaf kind: the pre-summed leaves are FP32 (column-major operands)
     size_t y_off;                // the Y leaves: workspace offset
     fmmhip_wsum_t wsum2;
 };

@@ -1065,6 +1065,9 @@ int launch_batch_typed(const KArgs* probs, const BatchItem* items, const int* qo
 int fmmk_launch_d64(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
 int fmmk_launch_d_f32leaf(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg,
                           hipStream_t st);
+// FP32 pre-summed leaves of FP64 operands, FP32 MFMA leaf, FP64 C or slots
+// (column-major A, B, C; shapes 0, 2 and 3)
+int fmmk_launch_f32leaves_d(const fmmk::KArgs& ka, int shape, int nwg, hipStream_t st);
 int fmmk_launch_s32(const fmmk::KArgs& ka, int shape, bool a_mn, bool b_mn, bool c_col, int nwg, hipStream_t st);
 // items: kBatchQueues queues, queue q at items[qoff[q] .. qoff[q+1]); heads: kBatchQueues counters
 int fmmk_launch_batch_d64(const fmmk::KArgs* probs, const fmmk::BatchItem* items, const int* qoff, int* heads,

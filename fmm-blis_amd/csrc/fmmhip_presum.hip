@@ -141,15 +141,15 @@ __device__ __forceinline__ void parts_meta(const fmm_entry_t* __restrict__ parts
 // every S_r at (c, s).  16 parts at a time, their loads in flight together:
 // every address is clamped into its part (or to X[0] for an empty part), and
 // elements outside the part are replaced by zero afterwards.
-template <typename T, int NPMAX, int TB>
+template <typename T, typename TO, int NPMAX, int TB>
 __device__ __forceinline__ void parts_unit(const T* __restrict__ X, long ld, const int* __restrict__ ptr,
                                            const fmm_entry_t* __restrict__ ent, int np, int R, int Lc, int ldo,
-                                           long plane, T* __restrict__ S, int s, int c, double (*xs)[TB],
+                                           long plane, TO* __restrict__ S, int s, int c, double (*xs)[TB],
                                            const long* p_off, const int* p_c, const int* p_s)
 {
     const int t = threadIdx.x;
     typename G1<T>::c* __restrict__ Xg = (typename G1<T>::c*)X;
-    typename G1<T>::m* __restrict__ Sg = (typename G1<T>::m*)S;
+    typename G1<TO>::m* __restrict__ Sg = (typename G1<TO>::m*)S;
     typename G1<int>::c* __restrict__ ptrg = (typename G1<int>::c*)ptr;
     for (int p0 = 0; p0 < np; p0 += 16) {
         double xv[16];
@@ -172,17 +172,18 @@ __device__ __forceinline__ void parts_unit(const T* __restrict__ X, long ld, con
                 const fmm_entry_t en = entry_at(ent, e);
                 v = fma(en.coef, xs[en.part][t], v);
             }
-            Sg[r * plane + c + (long)s * ldo] = (T)v;
+            Sg[r * plane + c + (long)s * ldo] = (TO)v;
         }
     }
 }
 
-template <typename T, int NPMAX, int TB>
+// TO: the leaves' type (FP32 leaves of FP64 operands for the FP32-leaf kernel)
+template <typename T, typename TO, int NPMAX, int TB>
 __global__ __launch_bounds__(TB) void presum_parts_kernel(const T* __restrict__ X, long ld,
                                                           const int* __restrict__ ptr,
                                                           const fmm_entry_t* __restrict__ ent,
                                                           const fmm_entry_t* __restrict__ parts, int np, int R,
-                                                          int swap, int Lc, int Ls, int ldo, T* __restrict__ S)
+                                                          int swap, int Lc, int Ls, int ldo, TO* __restrict__ S)
 {
     __shared__ double xs[NPMAX][TB];
     __shared__ long p_off[NPMAX];
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(TB) void presum_parts_kernel(const T* __restrict__ 
     for (long blk = blockIdx.x; blk < nbc * Ls; blk += gridDim.x) {
         const int s = (int)(blk / nbc);
         const int c = (int)(blk - (long)s * nbc) * TB + threadIdx.x;
-        parts_unit<T, NPMAX, TB>(X, ld, ptr, ent, np, R, Lc, ldo, plane, S, s, c, xs, p_off, p_c, p_s);
+        parts_unit<T, TO, NPMAX, TB>(X, ld, ptr, ent, np, R, Lc, ldo, plane, S, s, c, xs, p_off, p_c, p_s);
     }
 }
 
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(TB) void presum_parts_batch_kernel(const fmmhip_pre
     for (long u = b - boff[lo]; u < nbc * sg.Ls; u += nbl) {
         const int s = (int)(u / nbc);
         const int c = (int)(u - (long)s * nbc) * TB + threadIdx.x;
-        parts_unit<T, NPMAX, TB>((const T*)sg.X, sg.ld, sg.ptr, sg.ent, sg.np, sg.R, sg.Lc, sg.ldo, plane, (T*)sg.S, s,
+        parts_unit<T, T, NPMAX, TB>((const T*)sg.X, sg.ld, sg.ptr, sg.ent, sg.np, sg.R, sg.Lc, sg.ldo, plane, (T*)sg.S, s,
                                  c, xs, p_off, p_c, p_s);
     }
 }
@@ -323,29 +324,30 @@ __global__ __launch_bounds__(TB) void presum_stage_kernel(const T* __restrict__ 
     }
 }
 
-template <typename T, int NPMAX, int TB>
+template <typename T, typename TO, int NPMAX, int TB>
 int launch_parts(const fmmhip_presum_seg_t* g, hipStream_t st)
 {
     const long nbc = (g->Lc + TB - 1) / TB;
     long blocks = nbc * g->Ls;
     if (blocks <= 0) return FMMHIP_SUCCESS;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL((presum_parts_kernel<T, NPMAX, TB>), dim3((unsigned)blocks), dim3(TB), 0, st, (const T*)g->X,
-                       g->ld, g->ptr, g->ent, g->parts, g->np, g->R, g->swap, g->Lc, g->Ls, g->ldo, (T*)g->S);
+    hipLaunchKernelGGL((presum_parts_kernel<T, TO, NPMAX, TB>), dim3((unsigned)blocks), dim3(TB), 0, st,
+                       (const T*)g->X, g->ld, g->ptr, g->ent, g->parts, g->np, g->R, g->swap, g->Lc, g->Ls, g->ldo,
+                       (TO*)g->S);
     return hipGetLastError() == hipSuccess ? FMMHIP_SUCCESS : FMMHIP_ERR_HIP;
 }
 
-template <typename T>
+template <typename T, typename TO = T>
 int launch_parts_typed(const fmmhip_presum_seg_t* g, hipStream_t st)
 {
     // LDS per block = NPMAX * TB * 8 bytes: 32 KB, 64 KB, 40 KB, 64 KB,
     // 128 KB (36 parts, 2-level <2,3,2> and <3,2,2>: the 40-part form holds
     // four blocks per CU where the 64-part one held two)
-    if (g->np <= 16) return launch_parts<T, 16, 256>(g, st);
-    if (g->np <= 32) return launch_parts<T, 32, 256>(g, st);
-    if (g->np <= 40) return launch_parts<T, 40, 128>(g, st);
-    if (g->np <= 64) return launch_parts<T, 64, 128>(g, st);
-    if (g->np <= 256) return launch_parts<T, 256, 64>(g, st);
+    if (g->np <= 16) return launch_parts<T, TO, 16, 256>(g, st);
+    if (g->np <= 32) return launch_parts<T, TO, 32, 256>(g, st);
+    if (g->np <= 40) return launch_parts<T, TO, 40, 128>(g, st);
+    if (g->np <= 64) return launch_parts<T, TO, 64, 128>(g, st);
+    if (g->np <= 256) return launch_parts<T, TO, 256, 64>(g, st);
     return FMMHIP_ERR_UNSUPPORTED;
 }
 
@@ -356,6 +358,14 @@ extern "C" int fmmhip_presum_seg(int is_double, const fmmhip_presum_seg_t* seg, 
     if (!seg || !seg->parts || seg->np <= 0) return FMMHIP_ERR_INVALID;
     return is_double ? launch_parts_typed<double>(seg, (hipStream_t)stream)
                      : launch_parts_typed<float>(seg, (hipStream_t)stream);
+}
+
+// FP64 operand, FP32 leaves: each FP64 sum rounded once to FP32 (the value
+// the FP32-leaf kernel would stage from an FP64 leaf)
+extern "C" int fmmhip_presum_seg_d2s(const fmmhip_presum_seg_t* seg, void* stream)
+{
+    if (!seg || !seg->parts || seg->np <= 0) return FMMHIP_ERR_INVALID;
+    return launch_parts_typed<double, float>(seg, (hipStream_t)stream);
 }
 
 extern "C" int fmmhip_presum_stage(int is_double, const void* X, long ld, void* S, int swap, int Lc, int Ls, int ldo,
