@@ -531,50 +531,49 @@ extern "C" int fmmhip_batch_plan(fmmhip_batch_t b)
     //    batch's workspace would pass the bound.  Both give the same C.
     size_t ws_total = 0;
     std::vector<size_t> ws_off(np, 0);
-    int pack_fmm = 2;
-  prepare_all:
-    ws_total = 0;
-    b->est_ms = 0.0;
-    for (size_t i = 0; i < np && !rc; i++) {
-        fmmhip_batch_s::Prob& p = b->probs[i];
-        double est = 0.0;
-        TunerParams tp;
-        {
-            std::lock_guard<std::mutex> lk(g_tuner_mu);
-            tp = g_tp;
-        }
-        const bool work = p.m > 0 && p.n > 0 && p.k > 0;
-        p.cand = 0;
-        p.tile = 0;
-        if (work && p.force < 0 && p.force_tile < 0) {
-            p.cand = choose(p.dbl, p.m, p.n, p.k, false, &est, &p.tile);
-        } else if (work) {
-            p.cand = p.force >= 0 ? p.force : choose(p.dbl, p.m, p.n, p.k, false, nullptr);
-            if (p.force_tile >= 0) {
-                p.tile = p.force_tile;
-                est = est_ms(tp, p.dbl, p.cand, p.m, p.n, p.k, false, p.tile);
-            } else {
-                p.tile = best_tile(tp, p.dbl, p.cand, p.m, p.n, p.k, false, &est);
+    // pass 1 with the products materialised; pass 2, pre-summed, only when
+    // the first one's workspace passes the bound
+    for (int pack_fmm = 2; pack_fmm >= 1; pack_fmm--) {
+        ws_total = 0;
+        b->est_ms = 0.0;
+        for (size_t i = 0; i < np && !rc; i++) {
+            fmmhip_batch_s::Prob& p = b->probs[i];
+            double est = 0.0;
+            TunerParams tp;
+            {
+                std::lock_guard<std::mutex> lk(g_tuner_mu);
+                tp = g_tp;
+            }
+            const bool work = p.m > 0 && p.n > 0 && p.k > 0;
+            p.cand = 0;
+            p.tile = 0;
+            if (work && p.force < 0 && p.force_tile < 0) {
+                p.cand = choose(p.dbl, p.m, p.n, p.k, false, &est, &p.tile);
+            } else if (work) {
+                p.cand = p.force >= 0 ? p.force : choose(p.dbl, p.m, p.n, p.k, false, nullptr);
+                if (p.force_tile >= 0) {
+                    p.tile = p.force_tile;
+                    est = est_ms(tp, p.dbl, p.cand, p.m, p.n, p.k, false, p.tile);
+                } else {
+                    p.tile = best_tile(tp, p.dbl, p.cand, p.m, p.n, p.k, false, &est);
+                }
+            }
+            b->est_ms += est;
+            b->prep[i] = Prepared();
+            rc = prepare_call(b->algs[p.tile][p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1,
+                              p.lda, p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? pack_fmm : 0,
+                              p.cand > 0 ? -1 : 1, b->stream, &b->prep[i]);
+            // the batched kernel is instantiated for column-major operands only
+            if (!rc && !b->prep[i].nothing && !b->prep[i].scale_only &&
+                !(b->prep[i].a_mn && !b->prep[i].b_mn && b->prep[i].c_col))
+                rc = FMMHIP_ERR_UNSUPPORTED;
+            if (!rc && b->prep[i].presum) {
+                ws_off[i] = ws_total;
+                ws_total += (b->prep[i].ws_bytes + 255) & ~(size_t)255;
             }
         }
-        b->est_ms += est;
-        b->prep[i] = Prepared();
-        rc = prepare_call(b->algs[p.tile][p.cand], p.dbl ? Kind::D64 : Kind::S32, p.m, p.n, p.k, p.alpha, p.A, 1, p.lda,
-                          p.B, 1, p.ldb, p.beta, p.C, 1, p.ldc, p.cand > 0 ? pack_fmm : 0, p.cand > 0 ? -1 : 1,
-                          b->stream, &b->prep[i]);
-        // the batched kernel is instantiated for column-major operands only
-        if (!rc && !b->prep[i].nothing && !b->prep[i].scale_only &&
-            !(b->prep[i].a_mn && !b->prep[i].b_mn && b->prep[i].c_col))
-            rc = FMMHIP_ERR_UNSUPPORTED;
-        if (!rc && b->prep[i].presum) {
-            ws_off[i] = ws_total;
-            ws_total += (b->prep[i].ws_bytes + 255) & ~(size_t)255;
-        }
-    }
-    if (rc) return rc;
-    if (pack_fmm == 2 && !((double)ws_total < fmmhip_ws_bound())) {
-        pack_fmm = 1;
-        goto prepare_all;
+        if (rc) return rc;
+        if ((double)ws_total < fmmhip_ws_bound()) break;
     }
     if (ws_total) {
         if (hipMalloc(&b->ws, ws_total) != hipSuccess) {
