@@ -96,25 +96,29 @@ int upload_list(DevList& l, int R, hipStream_t stream)
 }
 
 // Whether the pre-summed leaves and the product slots of geometry g are laid
-// out for whole 128 x 128 tiles: column-major A, B and C whose leaf k is a
-// whole number of k-steps of either fast kernel (32) and whose leaf m or n
-// is not a multiple of 128.  A's leaves then take 128-aligned columns, the
-// slots 128-aligned rows and columns, and B's leaves a tail after the last
-// one, so that a kernel running whole tiles (the fast kernel) reads and
-// writes inside the workspace.  The padding rows of A and columns of B it
-// reads (never written; B's are the next leaf's first columns) meet only
-// the padding rows and columns of the slots, which nothing reads; k is not
-// padded, so every product of the valid region is the unpadded one.
+// out for whole 128 x 128 tiles and whole FP64 fast-kernel k-steps (16):
+// column-major A, B and C whose leaf m or n is not a multiple of 128 or
+// whose leaf k is not a multiple of 16.  A's leaves then take 128-aligned
+// columns and up16(k) of them, B's leaves up16(k) rows and a tail after the
+// last one, the slots 128-aligned rows and columns, so that
+// a kernel running whole tiles and k-steps (the fast kernel) reads and
+// writes inside the workspace.  The summation runs over the padded k (its
+// loads are clamped into the parts and zero outside them), so the k
+// padding holds zeros and adds nothing to a product; the padding rows of A
+// (never written) and columns of B (the next leaf's first columns) meet
+// only the padding rows and columns of the slots, which nothing reads.
 bool tile_padded(const fmm_geom_t& g)
 {
-    return g.rsA == 1 && g.rsB == 1 && g.rsC == 1 && g.lk > 0 && g.lk % 32 == 0 &&
-           (g.lm % fmmk::BM != 0 || g.ln % fmmk::BN != 0);
+    return g.rsA == 1 && g.rsB == 1 && g.rsC == 1 && g.lk > 0 &&
+           (g.lm % fmmk::BM != 0 || g.ln % fmmk::BN != 0 || g.lk % fmmk::BK != 0);
 }
 long up_tile(long x) { return (x + fmmk::BM - 1) / fmmk::BM * fmmk::BM; }
+long up_kstep(long k) { return (k + fmmk::BK - 1) / fmmk::BK * fmmk::BK; }
 
 // Workspace leaf layout of the pre-summed mode for one operand: R leaves of
 // Lc x Ls (contiguous dim c), leading dimension ldo = Lc rounded up to even
-// (to 128 for A's leaves of a tile-padded geometry), then `tail` elements.
+// (tile-padded geometries: A's ldo up to 128 and its Ls, B's Lc = ldo, to
+// whole k-steps), then `tail` elements.
 struct WsLayout {
     int Lc, Ls, ldo;
     long tail;
@@ -126,8 +130,12 @@ WsLayout ws_layout_a(const fmm_geom_t& g, bool a_mn)
     WsLayout w;
     w.Lc = (int)(a_mn ? g.lm : g.lk);
     w.Ls = (int)(a_mn ? g.lk : g.lm);
-    w.ldo = a_mn && tile_padded(g) ? (int)up_tile(w.Lc) : (w.Lc + 1) & ~1;
+    w.ldo = (w.Lc + 1) & ~1;
     w.tail = 0;
+    if (a_mn && tile_padded(g)) {
+        w.ldo = (int)up_tile(g.lm);
+        w.Ls = (int)up_kstep(g.lk);
+    }
     return w;
 }
 
@@ -137,7 +145,11 @@ WsLayout ws_layout_b(const fmm_geom_t& g, bool b_mn)
     w.Lc = (int)(b_mn ? g.ln : g.lk);
     w.Ls = (int)(b_mn ? g.lk : g.ln);
     w.ldo = (w.Lc + 1) & ~1;
-    w.tail = !b_mn && tile_padded(g) ? (up_tile(g.ln) - g.ln) * w.ldo : 0;
+    w.tail = 0;
+    if (!b_mn && tile_padded(g)) {
+        w.Lc = w.ldo = (int)up_kstep(g.lk);
+        w.tail = (up_tile(g.ln) - g.ln) * w.ldo;
+    }
     return w;
 }
 
@@ -1042,9 +1054,11 @@ int prepare_once(fmmhip_alg_t alg, Kind kind, long m, long n, long k, double alp
         ka.csC = sl.swap ? 1 : sl.ldm;
         ka.min_cm = (int)g.lm; ka.min_cn = (int)g.ln;
         if (tile_padded(g) && a_mn && !b_mn && rsC == 1 && T == fmmk::BM) {
-            // whole tiles over the padded leaves and slots (same tile grid)
+            // whole tiles and k-steps over the padded leaves and slots (same
+            // tile grid; the extra k-steps multiply zeros)
             ka.lm = ka.min_m = ka.min_cm = (int)up_tile(g.lm);
             ka.ln = ka.min_n = ka.min_cn = (int)up_tile(g.ln);
+            ka.lk = ka.min_k = (int)up_kstep(g.lk);
         }
         ka.alpha = 1.0;
         ka.beta = 0.0;
