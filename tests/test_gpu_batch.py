@@ -295,11 +295,13 @@ def test_fast_kernel_fp32_equals_shape0(case):
 
 
 # (dtype or leaf, m, n, k, levels): products materialised with leaf m or n
-# not a multiple of 128 and leaf k a multiple of 32, so the leaves and slots
-# are padded to whole tiles; the few tiles per product also make the launch
-# deal whole product groups to the XCDs
+# not a multiple of 128, or leaf k not a multiple of 16 (500, 250: padded to
+# 512, 256 with zeros the summation writes), so the leaves and slots are
+# padded to whole tiles and k-steps; the few tiles per product also make the
+# launch deal whole product groups to the XCDs
 PADDED = [("f8", 600, 520, 512, 1), ("f8", 1000, 1000, 1024, 2), ("f8/f32", 600, 520, 512, 1),
-          ("f4", 700, 900, 1024, 2), ("f8", 1400, 300, 768, 1)]
+          ("f4", 700, 900, 1024, 2), ("f8", 1400, 300, 768, 1), ("f8", 1000, 1000, 1000, 1),
+          ("f8", 1000, 900, 1000, 2), ("f8/f32", 1000, 700, 1000, 1), ("f4", 1000, 1000, 1000, 1)]
 
 
 @pytest.mark.parametrize("case", PADDED, ids=lambda c: "-".join(map(str, c)).replace("/", "_"))
