@@ -248,14 +248,18 @@ long fmmhip_alg_brent_violations(fmmhip_alg_t alg);
  * parts ahead), 2 = 64 x 64 tiles, 256 threads (column-major A, B and C
  * without the classical k-split; other calls take shape 0), 3 = the fast
  * FP64 kernel (128 x 128 tiles, 256 threads, fragments read one sub-step
- * ahead) for single-part operands -- classical, pre-summed, products
- * materialised -- with column-major A, B and C whose leaf extents are
- * multiples of 128 (k of 16) and every tile inside its parts (other calls
- * take shape 0), -1 = chosen per call (default: 3 where it applies, else 0,
- * or 1 for the fused pack mode).  The result does not depend on the shape
- * (every element's sums run in the same order).  The batch's tuner chooses
- * between shapes 0 and 2 per problem.  fmmhip_alg_last_shape reports the
- * shape the last FP64/FP32 call launched (-1 before the first). */
+ * ahead; FP32 leaf and FP32 operands: k-steps of 32) for single-part
+ * operands -- classical, pre-summed, products materialised -- with
+ * column-major A, B and C whose leaf extents are multiples of 128 (k of the
+ * k-step) and every tile inside its parts; with the products materialised
+ * the library pads the leaves and slots in its workspace to whole tiles and
+ * k-steps, so every such call qualifies (other calls take shape 0), -1 =
+ * chosen per call (default: 3 where it applies, else 0, or 1 for the fused
+ * pack mode).  The result does not depend on the shape (every element's
+ * sums run in the same order).  The batch's tuner chooses between 128 x 128
+ * tiles (shape 0, interior tiles on the fast kernel) and shape 2 per
+ * problem.  fmmhip_alg_last_shape reports the shape the last FP64/FP32 call
+ * launched (-1 before the first). */
 int  fmmhip_alg_set_shape(fmmhip_alg_t alg, int shape);
 int  fmmhip_alg_last_shape(fmmhip_alg_t alg, int* shape);
 /* How the U/V-weighted operand sums are formed and the W-weighted products
