@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(ROOT, "fmm-blis_amd"))
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 matrix peak
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+INT8_PEAK_TOPS = 5000.0       # dense int8 MFMA, tera-operations per second
 SWEEP_TOL = 5e-15            # FP64 FMM normalised error bound of the sweep (two levels of the deepest table)
 
 
@@ -300,15 +301,16 @@ def run_sweep(args, F):
 
 def run_precision(args, F):
     """The precision form: classical, 1- and 2-level Strassen with FP64,
-    FP32 and split BF16 leaves at 8192^3 and 16384^3 -- bf16x3 (four planes,
-    ten plane products), bf16x3_8 (three planes, eight) and bf16x3_6 (three
-    planes, six) -- on uniform[-1,1] inputs and
+    FP32, split BF16 and int8-slice leaves at 8192^3 and 16384^3 -- bf16x3
+    (four planes, ten plane products), bf16x3_8 (three planes, eight),
+    bf16x3_6 (three planes, six) and int8x4 (four int8 slices, ten slice
+    products) -- on uniform[-1,1] inputs and
     on the stress variant (rows of A and columns of B scaled by
     10^U(-4,4)); effective GFLOPS and the normalised error against the
     classical FP64 product of the same operands."""
     import numpy as np
     leaves = [("f64", F.LEAF_NATIVE), ("f32", F.LEAF_F32), ("bf16x3", F.LEAF_BF16X3), ("bf16x3_8", F.LEAF_BF16X3_8),
-              ("bf16x3_6", F.LEAF_BF16X3_6)]
+              ("bf16x3_6", F.LEAF_BF16X3_6), ("int8x4", F.LEAF_INT8X4)]
     rows = []
     for n in (8192, 16384):
         for variant in ("uniform", "stress"):
@@ -362,7 +364,8 @@ def run_precision(args, F):
         "dtype": "f64 operands; f64/f32/bf16 leaves",
         "data": "synthetic (SplitMix64 uniform[-1,1], seed 0; stress: rows of A, columns of B x 10^U(-4,4), seed 5)",
         "config": {"workload": "C = A*B, m=n=k in {8192, 16384}, classical / 222 L1 / 222 L2 x leaves f64, f32, "
-                               "bf16x3 (4 planes, 10 plane products), bf16x3_8 (3, 8), bf16x3_6 (3, 6)",
+                               "bf16x3 (4 planes, 10 plane products), bf16x3_8 (3, 8), bf16x3_6 (3, 6), "
+                               "int8x4 (4 int8 slices, 10 slice products)",
                    "best": f"{best['alg']} L{best['levels']} {best['leaf']}",
                    "worst_error": {f"{a} L{l} {f}": e for (a, l, f), e in worst.items()}, "results": rows},
     }
@@ -462,7 +465,7 @@ def main():
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--alg", default="222", help="222, 232, 322, 333, 444 or classical")
     ap.add_argument("--levels", type=int, default=2)
-    ap.add_argument("--leaf", default="f64", choices=["f64", "f32", "bf16x3", "bf16x3_8", "bf16x3_6"])
+    ap.add_argument("--leaf", default="f64", choices=["f64", "f32", "bf16x3", "bf16x3_8", "bf16x3_6", "int8x4"])
     ap.add_argument("--pack", type=int, default=-1,
                     help="0 fused, 1 pre-summed, 2 pre-summed with the products materialised, -1 automatic")
     ap.add_argument("--classical-steps", type=int, default=3)
@@ -517,7 +520,7 @@ def main():
     alg.set_pack(args.pack)
     info = alg.info()
     leaf = {"f64": F.LEAF_NATIVE, "f32": F.LEAF_F32, "bf16x3": F.LEAF_BF16X3, "bf16x3_8": F.LEAF_BF16X3_8,
-            "bf16x3_6": F.LEAF_BF16X3_6}[args.leaf]
+            "bf16x3_6": F.LEAF_BF16X3_6, "int8x4": F.LEAF_INT8X4}[args.leaf]
     tag = f"{args.alg}L{args.levels}_{args.leaf}_{n}"
 
     A = F.DeviceMatrix(m, k).fill_uniform(0, 0)
@@ -614,36 +617,43 @@ def main():
         Cf.free()
 
     # the precision form beside the FP64 line, on the same operands: the
-    # BF16x3 leaf (four planes, ten plane products) with this run's
-    # algorithm and with the classical one, each with its error against the
-    # classical FP64 product
+    # BF16x3 leaf (four planes, ten plane products) and the INT8x4 leaf
+    # (four int8 slices, ten slice products) with this run's algorithm and
+    # with the classical one, each with its error against the classical
+    # FP64 product
     bf3 = None
     if args.leaf == "f64" and not args.no_bf16x3 and not args.no_check:
         bf3 = []
         nAB = F.fro(A) * F.fro(B)
-        for label, a_run in ((f"bf16x3 leaf, {args.alg} L{args.levels}", alg),
-                             ("bf16x3 leaf, classical", F.Alg(classical=True))):
+        for lname, lid, desc, label, a_run in (
+                ("bf16x3", F.LEAF_BF16X3, "4 planes, 10 plane products", f"{args.alg} L{args.levels}", alg),
+                ("bf16x3", F.LEAF_BF16X3, "4 planes, 10 plane products", "classical", F.Alg(classical=True)),
+                ("int8x4", F.LEAF_INT8X4, "4 int8 slices, 10 slice products", f"{args.alg} L{args.levels}", alg),
+                ("int8x4", F.LEAF_INT8X4, "4 int8 slices, 10 slice products", "classical",
+                 F.Alg(classical=True))):
             Cb = F.DeviceMatrix(m, n).fill_uniform(0, 2)
-            F.dgemm_fmm(a_run, m, n, k, 1.0, A, B, 1.0, Cb, leaf=F.LEAF_BF16X3)
+            F.dgemm_fmm(a_run, m, n, k, 1.0, A, B, 1.0, Cb, leaf=lid)
             F.sync()
             b_err = F.fro_diff(Cb, Cr) / nAB
             be = [(F.Event(), F.Event()) for _ in range(3)]
             for e0, e1 in be:
                 e0.record()
-                F.dgemm_fmm(a_run, m, n, k, 1.0, A, B, 1.0, Cb, leaf=F.LEAF_BF16X3)
+                F.dgemm_fmm(a_run, m, n, k, 1.0, A, B, 1.0, Cb, leaf=lid)
                 e1.record()
             F.sync()
             b_ms = sum(e0.elapsed_ms(e1) for e0, e1 in be) / len(be)
-            bf3.append({"leaf": f"{label} (4 planes, 10 plane products)",
+            bf3.append({"leaf": f"{lname} leaf, {label} ({desc})",
                         "ms": round(b_ms, 3), "effective_gflops": round(2.0 * m * n * k / (b_ms * 1e-3) / 1e9, 1),
                         "rel_err_vs_classical": b_err, "meets_1e-10": b_err <= 1e-10})
             Cb.free()
             F.check(F.lib().fmmhip_workspace_release(), "workspace_release")
 
     fl = leaf_flops(info, m, n, k)
-    peak = {"f64": FP64_PEAK_TFLOPS, "f32": FP32_PEAK_TFLOPS}.get(args.leaf, BF16_PEAK_TFLOPS)
-    # the split BF16 leaves run 10, 8 or 6 bf16 plane products per leaf product
-    mfma_fl = fl * {"bf16x3": 10, "bf16x3_8": 8, "bf16x3_6": 6}.get(args.leaf, 1)
+    peak = {"f64": FP64_PEAK_TFLOPS, "f32": FP32_PEAK_TFLOPS, "int8x4": INT8_PEAK_TOPS}.get(args.leaf,
+                                                                                         BF16_PEAK_TFLOPS)
+    # the split BF16 leaves run 10, 8 or 6 bf16 plane products per leaf
+    # product, the INT8x4 leaf 10 int8 slice products (operations, not flops)
+    mfma_fl = fl * {"bf16x3": 10, "bf16x3_8": 8, "bf16x3_6": 6, "int8x4": 10}.get(args.leaf, 1)
     achieved = mfma_fl / (kern_ms * 1e-3) / 1e12
     pack = {0: "fused", 1: "pre-summed", 2: "pre-summed, products materialised"}[pack_mode]
     # the counter summary of this workload and mode (profiles/traffic_<tag>.txt)
@@ -651,6 +661,8 @@ def main():
           f"_{args.leaf}_{n}"
     if args.leaf.startswith("bf16"):
         pack = "bf16 planes (split stage)" + (", products materialised" if pack_mode == 2 else "")
+    if args.leaf == "int8x4":
+        pack = "int8 slices (row-maximum and slice stages)" + (", products materialised" if pack_mode == 2 else "")
     out = {
         "metric": "effective FP64 GFLOPS (2mnk/t) FMM vs classical at 16384^3; p99 GEMM latency",
         "value": round(eff, 1),
@@ -678,9 +690,11 @@ def main():
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3),
                        "mean": round(sum(lat) / len(lat), 3)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+                     "unit": "TOP/s" if args.leaf == "int8x4" else "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": read_traffic(tag),
                      "kernel": ("bf16x3_gemm_kernel" if args.leaf.startswith("bf16") else
+                                "int8x4_gemm_kernel" if args.leaf == "int8x4" else
                                 "fmm_fast_kernel" if alg.last_shape() == 3 else "fmm_gemm_kernel"),
                      "kernel_ms": round(kern_ms, 3),
                      "mfma_flops_per_launch": mfma_fl, "effective_over_peak": round(eff / 1e3 / peak, 4)},

@@ -1499,8 +1499,10 @@ extern "C" int fmmhip_alg_pack_for(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m,
     if (!alg || !pack || m < 0 || n < 0 || k < 0) return FMMHIP_ERR_INVALID;
     int rc = fmmhip_init();
     if (rc) return rc;
-    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6 || leaf == FMMHIP_LEAF_BF16X3_8) {
-        // planes always staged; products materialised as bf16x3_gemm decides
+    if (leaf == FMMHIP_LEAF_BF16X3 || leaf == FMMHIP_LEAF_BF16X3_6 || leaf == FMMHIP_LEAF_BF16X3_8 ||
+        leaf == FMMHIP_LEAF_INT8X4) {
+        // planes (slices) always staged; products materialised as
+        // bf16x3_gemm and int8x4_gemm decide
         bool wmat = alg->R > 1 && (alg->pack == 2 || alg->pack < 0);
         if (wmat && alg->pack < 0) {
             const double lm = (double)((m + alg->mt - 1) / alg->mt), ln = (double)((n + alg->nt - 1) / alg->nt);
@@ -1618,6 +1620,12 @@ extern "C" int fmmhip_dgemm_fmm(fmmhip_alg_t alg, fmmhip_leaf_t leaf, long m, lo
         // plane products: ten (four planes), six or eight (three planes)
         const int terms = leaf == FMMHIP_LEAF_BF16X3 ? 10 : leaf == FMMHIP_LEAF_BF16X3_6 ? 6 : 8;
         return bf16x3_gemm(alg, terms, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, st);
+    }
+    case FMMHIP_LEAF_INT8X4: {
+        int rc = fmmhip_init();
+        if (rc) return rc;
+        hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)fmmhip_default_stream();
+        return int8x4_gemm(alg, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, st);
     }
     default: return FMMHIP_ERR_UNSUPPORTED;
     }

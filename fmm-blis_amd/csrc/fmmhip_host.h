@@ -163,5 +163,9 @@ void timing_push(hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, bool presum);
 // `terms` (6 or 8) plane products.
 int bf16x3_gemm(fmmhip_alg_t alg, int terms, long m, long n, long k, double alpha, const double* A, long rsA, long csA,
                 const double* B, long rsB, long csB, double beta, double* C, long rsC, long csC, hipStream_t st);
+// The INT8x4 leaf (fmmhip_int8x4.hip): row maxima, int8 slices and the int8
+// MFMA GEMM with ten slice products.
+int int8x4_gemm(fmmhip_alg_t alg, long m, long n, long k, double alpha, const double* A, long rsA, long csA,
+                const double* B, long rsB, long csB, double beta, double* C, long rsC, long csC, hipStream_t st);
 
 #endif

@@ -26,7 +26,7 @@ BLIS_SUCCESS = -1
 BLIS_FAILURE = -2
 TRANS_BIT = 1 << 3
 
-LEAF_NATIVE, LEAF_F32, LEAF_BF16X3, LEAF_BF16X3_6, LEAF_BF16X3_8 = 0, 1, 2, 3, 4
+LEAF_NATIVE, LEAF_F32, LEAF_BF16X3, LEAF_BF16X3_6, LEAF_BF16X3_8, LEAF_INT8X4 = 0, 1, 2, 3, 4, 5
 
 STATUS = {0: "success", 1: "invalid argument", 2: "out of memory", 3: "HIP runtime error",
           4: "unsupported", 5: "no gfx950 device", 6: "coefficient file unreadable or malformed",

@@ -176,8 +176,13 @@ typedef enum {
                                  for one- and two-level Strassen at 8192^3-16384^3)   */
     FMMHIP_LEAF_BF16X3_6 = 3, /* three planes, six plane products (fastest; about 1e-10
                                  classical, 1e-9 two-level Strassen at 16384^3)       */
-    FMMHIP_LEAF_BF16X3_8 = 4  /* three planes, eight plane products (about 6e-11
+    FMMHIP_LEAF_BF16X3_8 = 4, /* three planes, eight plane products (about 6e-11
                                  classical, 4e-10 two-level Strassen at 16384^3)      */
+    FMMHIP_LEAF_INT8X4 = 5    /* FP64 operands, int8 MFMA leaf with exact int32 sums:
+                                 four int8 slices per value with one scale per row
+                                 (Ozaki-scheme slicing), ten slice products, FP64
+                                 recombination and writeback; a non-finite input makes
+                                 its row (column of B: column) of C NaN              */
 } fmmhip_leaf_t;
 
 /* Like bli_strassen_ab_ex with `nlevels` levels; level l uses algs[l]
