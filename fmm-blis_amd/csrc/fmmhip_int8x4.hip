@@ -14,7 +14,8 @@
 //
 //  1. row maxima (HBM-bound): for every product r, the U- (V-) weighted sum
 //     x of the operand's parts is formed in FP64 (as the pre-summed pack
-//     mode does) per block of 16 rows x 256 k, and the block's row maxima
+//     mode does) per block (64 rows x 64 k where rows are contiguous in
+//     memory, 16 x 256 where k is), and the block's row maxima
 //     of |x| go into MX[r][row] by an atomic maximum over the double's bits
 //     (a non-finite row stores a quiet NaN's bits, above every finite one).
 //  2. slices (HBM-bound): the same sums again; with mx = f 2^E the row
@@ -61,9 +62,17 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 template <int I> struct IntC { static constexpr int value = I; };
 
 constexpr int KSTEP = 64;        // k per MFMA stage (bytes per row and slice)
-constexpr int CHUNK = 256;       // k per split block
+constexpr int CHUNK = 256;       // k padding of the slices
 constexpr int ROW_PAD = 128;     // slice rows are padded to this
-constexpr int SROWS = 16;        // rows per split block
+// Split blocks: rows x k per block.  Rows contiguous in memory (A column-
+// major): 64 x 64, each k column 512 contiguous bytes and each slice's 64
+// rows x 64 bytes one contiguous write (16 x 256 blocks touched 256 lines
+// a leading dimension apart and re-read them from HBM for every product);
+// k contiguous: 16 x 256, each row 2 KB contiguous.
+template <bool ROWC> struct Blk {
+    static constexpr int SR = ROWC ? 64 : 16, CK = ROWC ? 64 : 256;
+    static constexpr int TPR = 256 / SR;       // threads per row in the row passes
+};
 constexpr int SLICES = 4;
 constexpr long KMAX = 32768;     // k per product for exact int32 sums
 constexpr unsigned long long kBadBits = 0x7FF8000000000000ULL;   // a quiet NaN: above +inf and every finite
@@ -81,22 +90,24 @@ struct SplitArgs {
     unsigned long long* MX;      // row maxima (double bits) [r][row]
     double* SC;                  // inverse scales [r][row]
     int R;
-    int nrc;                     // (row block, chunk) pairs: (Lp / SROWS) * (ldk / CHUNK)
+    int nrc;                     // (row block, k block) pairs: (Lp / SR) * (ldk / CK)
 };
 
 // Block ids are dealt so that the R blocks of one (row block, chunk) run
 // back to back on one XCD (ids go round-robin to the 8 XCDs): the parts
 // they all read come from that XCD's L2 after the first product.
+template <bool ROWC>
 __device__ __forceinline__ bool block_of(const SplitArgs& a, int& r, int& row0, int& k0, int& ch)
 {
+    typedef Blk<ROWC> K;
     const int xcd = (int)blockIdx.x & 7, t8 = (int)blockIdx.x >> 3;
     r = t8 % a.R;
     const int q = (t8 / a.R) * 8 + xcd;             // (row block, chunk) index
     if (q >= a.nrc) return false;
-    const int nrb = a.Lp / SROWS;
-    row0 = (q % nrb) * SROWS;
+    const int nrb = a.Lp / K::SR;
+    row0 = (q % nrb) * K::SR;
     ch = q / nrb;
-    k0 = ch * CHUNK;
+    k0 = ch * K::CK;
     return true;
 }
 
@@ -104,9 +115,11 @@ __device__ __forceinline__ bool block_of(const SplitArgs& a, int& r, int& row0, 
 // extents): one part at a time, the thread's elements of it loaded together
 // (clamped into the part, so every load is in bounds and unconditional).
 template <bool ROWC>
-__device__ __forceinline__ void sum_to_lds(const SplitArgs& a, int r, int row0, int k0, double (*T)[CHUNK + 1])
+__device__ __forceinline__ void sum_to_lds(const SplitArgs& a, int r, int row0, int k0,
+                                           double (*T)[Blk<ROWC>::CK + 1])
 {
-    constexpr int PER = SROWS * CHUNK / 256;
+    constexpr int SR = Blk<ROWC>::SR, CK = Blk<ROWC>::CK;
+    constexpr int PER = SR * CK / 256;
     const int t = threadIdx.x;
     double v[PER];
 #pragma unroll
@@ -122,44 +135,45 @@ __device__ __forceinline__ void sum_to_lds(const SplitArgs& a, int r, int row0, 
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int idx = t + 256 * i;
-            const int gi = row0 + (ROWC ? (idx % SROWS) : (idx / CHUNK));
-            const int gk = k0 + (ROWC ? (idx / SROWS) : (idx % CHUNK));
+            const int gi = row0 + (ROWC ? (idx % SR) : (idx / CK));
+            const int gk = k0 + (ROWC ? (idx / SR) : (idx % CK));
             x[i] = Xe[(long)min(gi, er - 1) * a.sr + (long)min(gk, ek - 1) * a.sk];
         }
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int idx = t + 256 * i;
-            const int gi = row0 + (ROWC ? (idx % SROWS) : (idx / CHUNK));
-            const int gk = k0 + (ROWC ? (idx / SROWS) : (idx % CHUNK));
+            const int gi = row0 + (ROWC ? (idx % SR) : (idx / CK));
+            const int gk = k0 + (ROWC ? (idx / SR) : (idx % CK));
             if (gi < er && gk < ek) v[i] = fma(en.coef, x[i], v[i]);
         }
     }
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         const int idx = t + 256 * i;
-        T[ROWC ? (idx % SROWS) : (idx / CHUNK)][ROWC ? (idx / SROWS) : (idx % CHUNK)] = v[i];
+        T[ROWC ? (idx % SR) : (idx / CK)][ROWC ? (idx / SR) : (idx % CK)] = v[i];
     }
 }
 
 template <bool ROWC>
 __global__ __launch_bounds__(256) void rowmax_kernel(SplitArgs a)
 {
-    __shared__ double T[SROWS][CHUNK + 1];
+    typedef Blk<ROWC> K;
+    __shared__ double T[K::SR][K::CK + 1];
     int r, row0, k0, ch;
-    if (!block_of(a, r, row0, k0, ch)) return;
+    if (!block_of<ROWC>(a, r, row0, k0, ch)) return;
     sum_to_lds<ROWC>(a, r, row0, k0, T);
     __syncthreads();
-    const int row = threadIdx.x >> 4, s = threadIdx.x & 15;
+    const int row = threadIdx.x / K::TPR, s = threadIdx.x % K::TPR;
     double mx = 0.0;
     int bad = 0;
 #pragma unroll
-    for (int j = 0; j < CHUNK / 16; j++) {
-        const double x = T[row][s + 16 * j];
+    for (int j = 0; j < K::CK / K::TPR; j++) {
+        const double x = T[row][s + K::TPR * j];
         mx = fmax(mx, fabs(x));
         bad |= !isfinite(x);
     }
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
+    for (int o = 1; o < K::TPR; o <<= 1) {
         mx = fmax(mx, __shfl_xor(mx, o));
         bad |= __shfl_xor(bad, o);
     }
@@ -170,27 +184,30 @@ __global__ __launch_bounds__(256) void rowmax_kernel(SplitArgs a)
 template <bool ROWC>
 __global__ __launch_bounds__(256) void slice_kernel(SplitArgs a)
 {
-    static_assert(SLICES * SROWS * CHUNK <= SROWS * (CHUNK + 1) * 8, "the slices fit in the sum's LDS");
-    __shared__ __attribute__((aligned(16))) double T[SROWS][CHUNK + 1];
+    typedef Blk<ROWC> K;
+    constexpr int SR = K::SR, CK = K::CK, TPR = K::TPR;
+    static_assert(SLICES * SR * CK <= SR * (CK + 1) * 8, "the slices fit in the sum's LDS");
+    static_assert(CK % KSTEP == 0 && (SLICES * SR * CK / 16) % 256 == 0, "whole 16-byte pieces per thread");
+    __shared__ __attribute__((aligned(16))) double T[SR][CK + 1];
     int r, row0, k0, ch;
-    if (!block_of(a, r, row0, k0, ch)) return;
+    if (!block_of<ROWC>(a, r, row0, k0, ch)) return;
     sum_to_lds<ROWC>(a, r, row0, k0, T);
     __syncthreads();
-    const int t = threadIdx.x, row = t >> 4, s = t & 15;
+    const int t = threadIdx.x, row = t / TPR, s = t % TPR;
     const double mx = __longlong_as_double((long long)a.MX[(long)r * a.Lp + row0 + row]);
     const bool bad = !isfinite(mx);
     int E = 0;
     double f = 0.5;
     if (!bad && mx > 0.0) f = frexp(mx, &E);        // mx = f 2^E, f in [1/2, 1)
     const double up = 127.0 / f;                     // |x| 2^-E up <= 127
-    double x[CHUNK / 16];
+    double x[CK / TPR];
 #pragma unroll
-    for (int j = 0; j < CHUNK / 16; j++) x[j] = T[row][s + 16 * j];
+    for (int j = 0; j < CK / TPR; j++) x[j] = T[row][s + TPR * j];
     __syncthreads();                                 // T is reused for the slices
-    int8_t* O = (int8_t*)&T[0][0];                   // O[g][row][CHUNK]
+    int8_t* O = (int8_t*)&T[0][0];                   // O[g][row][CK]
 #pragma unroll
-    for (int j = 0; j < CHUNK / 16; j++) {
-        const int kk = s + 16 * j;
+    for (int j = 0; j < CK / TPR; j++) {
+        const int kk = s + TPR * j;
         double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
         if (!bad) {
             // y - rint(y) is exact; a remainder of at most 1/2 times 254 is
@@ -203,24 +220,24 @@ __global__ __launch_bounds__(256) void slice_kernel(SplitArgs a)
             d2 = rint(r2);
             d3 = rint((r2 - d2) * 254.0);
         }
-        O[(0 * SROWS + row) * CHUNK + kk] = (int8_t)(int)d0;
-        O[(1 * SROWS + row) * CHUNK + kk] = (int8_t)(int)d1;
-        O[(2 * SROWS + row) * CHUNK + kk] = (int8_t)(int)d2;
-        O[(3 * SROWS + row) * CHUNK + kk] = (int8_t)(int)d3;
+        O[(0 * SR + row) * CK + kk] = (int8_t)(int)d0;
+        O[(1 * SR + row) * CK + kk] = (int8_t)(int)d1;
+        O[(2 * SR + row) * CK + kk] = (int8_t)(int)d2;
+        O[(3 * SR + row) * CK + kk] = (int8_t)(int)d3;
     }
     if (s == 0 && ch == 0) a.SC[(long)r * a.Lp + row0 + row] = bad ? __longlong_as_double((long long)kBadBits) : mx / 127.0;
     __syncthreads();
-    // write out: per (slice, k step) SROWS rows of 64 B, contiguous
+    // write out: per (slice, k step) SR rows of 64 B, contiguous
     const long plane = (long)a.ldk * a.Lp;
 #pragma unroll
-    for (int i = 0; i < SLICES * SROWS * CHUNK / 16 / 256; i++) {
+    for (int i = 0; i < SLICES * SR * CK / 16 / 256; i++) {
         const int idx = t + 256 * i;
-        const int g = idx / (SROWS * CHUNK / 16);
-        const int rem = idx % (SROWS * CHUNK / 16);
-        const int kbl = rem / (SROWS * 4);
-        const int rem2 = rem % (SROWS * 4);
+        const int g = idx / (SR * CK / 16);
+        const int rem = idx % (SR * CK / 16);
+        const int kbl = rem / (SR * 4);
+        const int rem2 = rem % (SR * 4);
         const int rr = rem2 >> 2, c = rem2 & 3;
-        const uint4 v = *(const uint4*)&O[(g * SROWS + rr) * CHUNK + kbl * KSTEP + c * 16];
+        const uint4 v = *(const uint4*)&O[(g * SR + rr) * CK + kbl * KSTEP + c * 16];
         int8_t* dst = a.P + ((long)r * SLICES + g) * plane + ((long)(k0 / KSTEP + kbl) * a.Lp + row0 + rr) * KSTEP + c * 16;
         *(uint4*)dst = v;
     }
@@ -534,7 +551,6 @@ int int8x4_gemm(fmmhip_alg_t alg, long m, long n, long k, double alpha, const do
     const int R = alg->R;
     const long LpA = up_to(g.lm, i8s::ROW_PAD), LpB = up_to(g.ln, i8s::ROW_PAD);
     const long ldk = up_to(g.lk, i8s::CHUNK);
-    const long nch = ldk / i8s::CHUNK;
     if (LpA > INT32_MAX || LpB > INT32_MAX || ldk > i8s::KMAX) return FMMHIP_ERR_UNSUPPORTED;
     const size_t slA = (size_t)R * i8s::SLICES * LpA * ldk, slB = (size_t)R * i8s::SLICES * LpB * ldk;
     const size_t rowA = (size_t)R * LpA * 8, rowB = (size_t)R * LpB * 8;
@@ -578,12 +594,17 @@ int int8x4_gemm(fmmhip_alg_t alg, long m, long n, long k, double alpha, const do
     const bool timed = timing_active();
     if (timed) timing_record(&e0, st);
     const dim3 blk(256);
-    if ((double)up_to(LpA / i8s::SROWS * nch, 8) * R > INT32_MAX ||
-        (double)up_to(LpB / i8s::SROWS * nch, 8) * R > INT32_MAX)
+    // (row block, k block) pairs of each operand for its block shape
+    auto nrc = [&](long Lp, bool rowc) {
+        return rowc ? Lp / i8s::Blk<true>::SR * (ldk / i8s::Blk<true>::CK)
+                    : Lp / i8s::Blk<false>::SR * (ldk / i8s::Blk<false>::CK);
+    };
+    const long nrcA = nrc(LpA, rsA == 1), nrcB = nrc(LpB, csB == 1);
+    if ((double)up_to(nrcA, 8) * R > INT32_MAX || (double)up_to(nrcB, 8) * R > INT32_MAX)
         return FMMHIP_ERR_UNSUPPORTED;
     sa.R = sb.R = R;
-    sa.nrc = (int)(LpA / i8s::SROWS * nch);
-    sb.nrc = (int)(LpB / i8s::SROWS * nch);
+    sa.nrc = (int)nrcA;
+    sb.nrc = (int)nrcB;
     // grid: (row block, chunk) pairs rounded up to 8 (one run per XCD) x products
     const dim3 gA((unsigned)(up_to(sa.nrc, 8) * R));
     const dim3 gB((unsigned)(up_to(sb.nrc, 8) * R));
