@@ -25,7 +25,8 @@
 //     every digit in [-127, 127] (a remainder is at most 1/2, times 254),
 //     written as four int8 planes; the row's inverse scale mx / 127 goes to
 //     SC[r][row] (NaN for a non-finite row, 0 for a zero row).
-//     Representation error: 1/2 of 254^-3 of 127, 2^-32 of the row maximum.
+//     Representation error: half a unit of d3, 0.5 / (127 * 254^3) = 2.4e-10
+//     (about 2^-32) of the row maximum.
 //  3. GEMM: one workgroup per 128 x 128 leaf tile of C, all R products; per
 //     64-wide k step the ten slice products of weight at least 254^-3 into
 //     four int32 accumulators, one per weight:
@@ -38,7 +39,7 @@
 //     scaled by the row's and the column's inverse scales, and the W
 //     epilogue scatters alpha*w*c into the C parts (race-free: the workgroup
 //     owns the tile in every part).  The dropped products (weights 254^-4
-//     and below) are each below 2^-32 of the row maximum times the column
+//     and below) are each below 254^-4 (about 2^-32) of the row maximum times the column
 //     maximum.
 //
 // Slices in HBM: P[r][g][kb][row][64 bytes] (kb = k/64): one k step of a

@@ -183,7 +183,7 @@ typedef enum {
                                  of the summed operand (Ozaki-scheme slicing), ten
                                  slice products, FP64 recombination and writeback
                                  (about 4e-12 classical, 4e-11 two-level Strassen at
-                                 16384^3; up to 5 * 2^-32 at k of a few); a
+                                 16384^3; up to 1.2e-9 at k of a few); a
                                  non-finite value makes its whole row NaN          */
 } fmmhip_leaf_t;
 

@@ -16,10 +16,11 @@ Bars (stated per test):
     parts) takes the two-level bound (measured 1.03e-10, 6.1x);
   * ragged shapes, alpha/beta, beta = 0 with a NaN-filled C (C not read),
     small-integer and uniform inputs: normalised error below BOUND_TERM =
-    5 * 2^-32 against the oracle (the classical leaf and 1- and 2-level
-    FMM): each slice product's term errs by at most 2^-32 of the row
+    5 e against the oracle, e = 0.5 / (127 * 254^3) = 2.4e-10 (about
+    2^-32; the classical leaf and 1- and 2-level FMM): each slice product's
+    term errs by at most e of the row
     maximum times the column maximum for each operand's representation and
-    3 * 2^-32 for the dropped slice products; over k terms of random sign
+    below 3 e for the dropped slice products; over k terms of random sign
     the normalised error falls about as 1/sqrt(k), so the precision target
     below holds at large k, while at k of a few it is this bound (3 x 13 x
     7 with small integers: 2.0e-10);
@@ -54,7 +55,7 @@ TOL = {"classical": 1.5 * MODEL_CLASSICAL, "222x1": 1.5 * 2.7 * MODEL_CLASSICAL,
 TOL_STRESS = {"classical": 1.5 * 2.4e-11, "222x2": 1.5 * 6.8 * 2.4e-11}
 TOL_ELEM = 1.5 * 2.8e-8
 TARGET = 1e-10     # the precision form's bound against the FP64 product
-BOUND_TERM = 5 * 2.0 ** -32
+BOUND_TERM = 5 * 0.5 / (127 * 254.0 ** 3)
 
 
 def say(*a):
